@@ -1,0 +1,174 @@
+/*
+ * pointneighbors_hip.h — C-ABI of the MI355X (gfx950) neighbourhood-search engine.
+ *
+ * This is the drop-in boundary under PointNeighbors.jl's GridNeighborhoodSearch +
+ * FullGridCellList + PrecomputedNeighborhoodSearch path.  Every entry point is plain C:
+ * device pointers, sizes and status codes; no language runtime types cross it.
+ * A Julia binding calls these through `ccall`; the Python host mirror in
+ * pointneighbors.jl_amd/pointneighbors calls them through ctypes (same convention).
+ *
+ * Conventions
+ *  - Coordinates are column-major D x N matrices (the reference's layout,
+ *    src/neighborhood_search.jl:24-26): point i occupies elements [D*i, D*i + D).
+ *  - Point indices crossing the ABI are 0-based unless an `index_base` argument says
+ *    otherwise (a Julia binding passes index_base = 1).
+ *  - All device work is enqueued on the grid's stream (pn_grid_set_stream).  Functions
+ *    documented as "synchronizes" wait for that stream before returning, like the
+ *    reference's KernelAbstractions.synchronize after every @threaded loop
+ *    (src/util.jl:170).
+ *  - Status codes: 0 = PN_OK; on failure the thread-local message is pn_last_error().
+ *  - A grid is not reentrant: calls on one grid are serialized by the caller.
+ */
+#ifndef POINTNEIGHBORS_HIP_H
+#define POINTNEIGHBORS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PN_ABI_VERSION 1
+
+enum pn_status {
+    PN_OK = 0,
+    PN_ERR_ARGUMENT = 1,       /* ArgumentError in the reference constructors */
+    PN_ERR_OUT_OF_BOUNDS = 2,  /* "particle coordinates are NaN or outside the domain
+                                  bounds of the cell list" (src/cell_lists/full_grid.jl:187-213) */
+    PN_ERR_CELL_FULL = 3,      /* "cell list is full. Use a larger `max_points_per_cell`."
+                                  (src/vector_of_vectors.jl:114-121) */
+    PN_ERR_QUERY_BOUNDS = 4,   /* BoundsError: a query point's neighbour cells leave the grid
+                                  (LinearIndices getindex, src/cell_lists/full_grid.jl:157-161) */
+    PN_ERR_NOT_INITIALIZED = 5,
+    PN_ERR_HIP = 6,            /* HIP runtime failure */
+    PN_ERR_LIST_FULL = 7,      /* a precomputed neighbour list exceeds max_neighbors
+                                  (pushat! overflow, src/vector_of_vectors.jl:79) */
+    PN_ERR_UNSUPPORTED = 8
+};
+
+enum pn_dtype { PN_FLOAT32 = 0, PN_FLOAT64 = 1 };
+
+/* Which points a traversal iterates over (the `x` / system_coords side). */
+enum pn_queries {
+    PN_QUERIES_GRID = 0,   /* x is y: the grid's own binned points */
+    PN_QUERIES_BINNED = 1  /* the point set last passed to pn_grid_bin_queries */
+};
+
+typedef struct pn_grid* pn_grid_t;
+
+/* Library identity.  No reference counterpart. */
+int pn_abi_version(void);
+const char* pn_last_error(void);
+
+/* ------------------------------------------------------------------------------------
+ * Grid lifetime.  Replaces FullGridCellList(; min_corner, max_corner, search_radius,
+ * max_points_per_cell) (src/cell_lists/full_grid.jl:48-82) together with
+ * GridNeighborhoodSearch{D}(; search_radius, cell_list) (src/nhs_grid.jl:77-129) and
+ * Adapt.adapt(backend, nhs) (src/gpu.jl:10-12).
+ *
+ * The caller computes the padded min corner and n_cells exactly as the reference
+ * constructor does and passes them here:
+ *   coord_dtype   element type of the coordinate matrices x and y
+ *   radius_dtype  type of search_radius (determines the distance type, src/nhs_grid.jl:547)
+ *   cell_dtype    promote(coord type, padded-corner type): the type of
+ *                 (x .- min_corner) ./ cell_size (src/cell_lists/full_grid.jl:93)
+ *   min_corner    padded min corner, values exact in cell_dtype
+ *   n_cells       cells per dimension (including the padding layer)
+ * ---------------------------------------------------------------------------------- */
+int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
+                   const double* min_corner, const int64_t* n_cells, double search_radius,
+                   int32_t max_points_per_cell, pn_grid_t* out);
+int pn_grid_destroy(pn_grid_t grid);
+/* stream: a hipStream_t (NULL = default stream). */
+int pn_grid_set_stream(pn_grid_t grid, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * initialize!/update! with the default ParallelUpdate strategy
+ * (src/nhs_grid.jl:220-225, :255-292, :470-477): bins the points of y selected by
+ * eachindex_y into cells with a counting sort.
+ *   y         device pointer, D x n_y of coord_dtype
+ *   sel_list  NULL: the selection is the range [sel_first, sel_first + sel_count);
+ *             else a device int32 array of sel_count 0-based indices (eachindex_y)
+ * pn_grid_update synchronizes and reports the reference's errors:
+ *   PN_ERR_OUT_OF_BOUNDS (first offending point via pn_grid_error), PN_ERR_CELL_FULL.
+ * pn_grid_update_async only enqueues; pn_grid_check later synchronizes and reports.
+ * ---------------------------------------------------------------------------------- */
+int pn_grid_update(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first,
+                   int64_t sel_count, const int32_t* sel_list);
+int pn_grid_update_async(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first,
+                         int64_t sel_count, const int32_t* sel_list);
+int pn_grid_check(pn_grid_t grid);
+/* After PN_ERR_OUT_OF_BOUNDS / PN_ERR_QUERY_BOUNDS: the offending 0-based point index and
+ * its 1-based cell coordinates (ndims entries, Int64 wrap as in src/util.jl:19-34). */
+int pn_grid_error(pn_grid_t grid, int64_t* point, int64_t* cell);
+
+/* Re-gathers the grid's cell-sorted copy of y from y (cells unchanged).  Needed when y
+ * changed in place without update!: the reference reads neighbour coordinates at
+ * traversal time (src/nhs_grid.jl:543-545) while cell membership stays as built. */
+int pn_grid_refresh(pn_grid_t grid, const void* y, int64_t n_y);
+
+/* Bins a query point set x (the `system_coords` of foreach_point_neighbor,
+ * src/neighborhood_search.jl:183-201) with the grid's geometry, restricted to `points`
+ * (same selection convention as pn_grid_update).  Synchronizes; PN_ERR_QUERY_BOUNDS when
+ * a query cell is outside 2:(n_d - 1). */
+int pn_grid_bin_queries(pn_grid_t grid, const void* x, int64_t n_x, int64_t sel_first,
+                        int64_t sel_count, const int32_t* sel_list);
+
+/* Counters of the last build (host values; synchronizes): number of binned points,
+ * number of non-empty cells, largest cell population. */
+int pn_grid_stats(pn_grid_t grid, int64_t* n_binned, int64_t* n_nonempty, int64_t* max_count);
+/* Copies the cell table of the grid (device -> device, enqueued): cell_start has
+ * prod(n_cells) + 1 entries (0-based offsets into the sorted order), sorted_index the
+ * original 0-based point index of every binned point in cell order (cells ascending,
+ * original index ascending within a cell).  Either pointer may be NULL. */
+int pn_grid_copy_cells(pn_grid_t grid, int32_t* cell_start, int32_t* sorted_index);
+
+/* ------------------------------------------------------------------------------------
+ * foreach_point_neighbor with a fixed functor menu (src/neighborhood_search.jl:183-201,
+ * hot loop src/nhs_grid.jl:519-575).  A pair (i, j) is visited iff j is a binned point
+ * in one of the 3^D cells around i's cell and d2 <= r*r with
+ * d2 = ((p1*p1 + p2*p2) + p3*p3), p = convert(T_r, x_i - y_j).
+ * Outputs are indexed by the original query index.  accumulate != 0 adds to the
+ * output (the `out[i] += ...` closure form); 0 overwrites.  Points outside the query
+ * selection are not touched.  All enqueue only.
+ * ---------------------------------------------------------------------------------- */
+/* Closure `n_neighbors[i] += 1` (benchmarks/count_neighbors.jl:24-27). */
+int pn_count_neighbors(pn_grid_t grid, int queries, int32_t* counts, int accumulate);
+/* mapreduce_neighbor(+, ...) of (j + index_base) (test/neighborhood_search.jl:389-399). */
+int pn_sum_neighbor_indices(pn_grid_t grid, int queries, int64_t* sums, int index_base,
+                            int accumulate);
+
+/* ------------------------------------------------------------------------------------
+ * PrecomputedNeighborhoodSearch lists as CSR (src/nhs_precomputed.jl:187-206):
+ *   pn_csr_count: row_ptr[n_rows + 1] (int32, device) from neighbour counts of the query
+ *     rows; returns nnz and the longest row (synchronizes).  PN_ERR_LIST_FULL when a
+ *     row exceeds max_neighbors (> 0), the reference's pushat! overflow.
+ *   pn_csr_fill: col_idx[nnz] with every row sorted ascending (sorteach!,
+ *     src/vector_of_vectors.jl:177-212), values j + index_base.
+ * n_rows is the number of columns of x (rows of points outside the selection are empty).
+ * ---------------------------------------------------------------------------------- */
+int pn_csr_count(pn_grid_t grid, int queries, int64_t n_rows, int32_t* row_ptr,
+                 int32_t max_neighbors, int64_t* nnz, int64_t* max_row);
+int pn_csr_fill(pn_grid_t grid, int queries, int64_t n_rows, const int32_t* row_ptr,
+                int32_t* col_idx, int index_base);
+
+/* ------------------------------------------------------------------------------------
+ * foreach_point_neighbor over precomputed lists (src/nhs_precomputed.jl:210-247): every
+ * stored neighbour is visited, without a radius test.  These take the CSR arrays and a
+ * stream directly, so they work after freeze_neighborhood_search dropped the grid
+ * (src/nhs_precomputed.jl:268-277).  Rows are selected like pn_grid_update's points.
+ * stored_base is the index base the lists were filled with.  Enqueue only.
+ * ---------------------------------------------------------------------------------- */
+int pn_list_count_neighbors(const int32_t* row_ptr, int64_t n_rows, int64_t sel_first,
+                            int64_t sel_count, const int32_t* sel_list, int32_t* counts,
+                            int accumulate, void* stream);
+int pn_list_sum_neighbor_indices(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
+                                 int stored_base, int64_t sel_first, int64_t sel_count,
+                                 const int32_t* sel_list, int64_t* sums, int index_base,
+                                 int accumulate, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POINTNEIGHBORS_HIP_H */

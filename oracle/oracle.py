@@ -1,0 +1,223 @@
+"""ctypes wrapper of the oracle (liboracle.so, built from pn_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+OK, ERR_ARG, ERR_OUT_OF_BOUNDS, ERR_CELL_FULL, ERR_QUERY_BOUNDS = 0, 1, 2, 3, 4
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, i64, dbl, i32, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, ctypes.c_int
+        L.orc_floor_to_int_f64.restype = i64
+        L.orc_floor_to_int_f64.argtypes = [dbl]
+        L.orc_floor_to_int_f32.restype = i64
+        L.orc_floor_to_int_f32.argtypes = [ctypes.c_float]
+        L.orc_fullgrid_geometry.restype = ci
+        L.orc_fullgrid_geometry.argtypes = [ci, ci, ci, vp, vp, dbl, vp, vp, vp]
+        L.orc_cell_coords.restype = None
+        L.orc_cell_coords.argtypes = [ci, ci, ci, vp, vp, dbl, vp]
+        L.orc_candidates_dict.restype = i64
+        L.orc_candidates_dict.argtypes = [ci, vp, i64, vp, dbl, vp]
+        L.orc_grid_create.restype = vp
+        L.orc_grid_create.argtypes = [ci, ci, ci, ci, vp, vp, dbl, i64]
+        L.orc_grid_destroy.restype = None
+        L.orc_grid_destroy.argtypes = [vp]
+        L.orc_grid_build.restype = ci
+        L.orc_grid_build.argtypes = [vp, vp, vp, i64, vp, vp]
+        L.orc_grid_lengths.restype = None
+        L.orc_grid_lengths.argtypes = [vp, vp]
+        L.orc_grid_cell.restype = i32
+        L.orc_grid_cell.argtypes = [vp, i64, vp]
+        L.orc_traverse.restype = ci
+        L.orc_traverse.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_trivial_count.restype = ci
+        L.orc_trivial_count.argtypes = [ci, ci, ci, vp, vp, i64, i64, dbl, vp, vp, vp]
+        L.orc_precomputed_lists.restype = ci
+        L.orc_precomputed_lists.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp]
+        L.orc_threaded_count_f32.restype = ci
+        L.orc_threaded_count_f32.argtypes = [vp, vp, i64, ci, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def code(dt) -> int:
+    return 1 if np.dtype(dt) == np.float64 else 0
+
+
+def floor_to_int(x, dtype=np.float64) -> int:
+    if np.dtype(dtype) == np.float32:
+        return lib().orc_floor_to_int_f32(ctypes.c_float(x))
+    return lib().orc_floor_to_int_f64(float(x))
+
+
+def fullgrid_geometry(user_min, user_max, radius, user_dtype=np.float64, r_dtype=np.float64):
+    """Returns (padded_min, padded_max, n_cells or None for a template, corner dtype)."""
+    nd = len(user_min)
+    umin = np.asarray(user_min, dtype=np.float64)
+    umax = np.asarray(user_max, dtype=np.float64)
+    omin, omax = np.zeros(nd), np.zeros(nd)
+    on = np.zeros(nd, dtype=np.int64)
+    tm = lib().orc_fullgrid_geometry(nd, code(user_dtype), code(r_dtype), _ptr(umin), _ptr(umax),
+                                     float(radius), _ptr(omin), _ptr(omax), _ptr(on))
+    corner_dtype = np.float64 if (code(user_dtype) or code(r_dtype)) else np.float32
+    return omin, omax, (None if tm < 0 else on), corner_dtype
+
+
+class Grid:
+    """Oracle GridNeighborhoodSearch + FullGridCellList (DVoV storage)."""
+
+    def __init__(self, ndims, min_corner, n_cells, radius, coord_dtype=np.float64,
+                 radius_dtype=np.float64, cell_dtype=np.float64, max_points_per_cell=100):
+        self.nd = ndims
+        self.tc = np.dtype(coord_dtype)
+        self.tr = np.dtype(radius_dtype)
+        self.tcell = np.dtype(cell_dtype)
+        mc = np.asarray(min_corner, dtype=np.float64)
+        nc = np.asarray(n_cells, dtype=np.int64)
+        self.n_cells = tuple(int(v) for v in nc)
+        self.ncells = int(np.prod(nc))
+        self.cap = max_points_per_cell
+        self.h = lib().orc_grid_create(ndims, code(coord_dtype), code(radius_dtype), code(cell_dtype),
+                                       _ptr(mc), _ptr(nc), float(radius), int(max_points_per_cell))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_grid_destroy(self.h)
+            self.h = None
+
+    def _pts(self, a):
+        a = np.ascontiguousarray(a, dtype=self.tc)
+        assert a.ndim == 2 and a.shape[1] == self.nd
+        return a
+
+    def build(self, y_nd, active=None):
+        """initialize!/update!; y_nd is (N, D).  Returns (status, bad_point, bad_cell)."""
+        y = self._pts(y_nd)
+        act = np.arange(y.shape[0], dtype=np.int64) if active is None else np.asarray(active, dtype=np.int64)
+        bp = ctypes.c_int64(-1)
+        bc = np.zeros(3, dtype=np.int64)
+        st = lib().orc_grid_build(self.h, _ptr(y), _ptr(act), len(act), ctypes.byref(bp), _ptr(bc))
+        self._y = y
+        return st, bp.value, tuple(bc[: self.nd])
+
+    def lengths(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        lib().orc_grid_lengths(self.h, _ptr(out))
+        return out
+
+    def cell(self, linear0):
+        out = np.zeros(self.cap, dtype=np.int32)
+        n = lib().orc_grid_cell(self.h, int(linear0), _ptr(out))
+        return out[:n].tolist()
+
+    def traverse(self, x_nd, y_nd, points=None, lists=False):
+        """Counts, index sums and (optionally) neighbour lists in traversal order."""
+        x = self._pts(x_nd)
+        y = self._pts(y_nd)
+        pts = np.arange(x.shape[0], dtype=np.int64) if points is None else np.asarray(points, dtype=np.int64)
+        counts = np.zeros(len(pts), dtype=np.int32)
+        sums = np.zeros(len(pts), dtype=np.int64)
+        bp = ctypes.c_int64(-1)
+        st = lib().orc_traverse(self.h, _ptr(x), _ptr(y), _ptr(pts), len(pts), _ptr(counts), _ptr(sums),
+                                None, None, ctypes.byref(bp))
+        if st != OK:
+            return st, bp.value, None, None, None
+        if not lists:
+            return st, -1, counts, sums, None
+        rowptr = np.zeros(len(pts) + 1, dtype=np.int64)
+        rowptr[1:] = np.cumsum(counts)
+        cols = np.zeros(max(int(rowptr[-1]), 1), dtype=np.int32)
+        st = lib().orc_traverse(self.h, _ptr(x), _ptr(y), _ptr(pts), len(pts), None, None, _ptr(rowptr),
+                                _ptr(cols), ctypes.byref(bp))
+        return st, -1, counts, sums, (rowptr, cols[: rowptr[-1]])
+
+    def precomputed(self, x_nd, y_nd, max_neighbors):
+        x = self._pts(x_nd)
+        y = self._pts(y_nd)
+        st, bp, counts, _, _ = self.traverse(x, y)
+        if st != OK:
+            return st, bp, None
+        rowptr = np.zeros(x.shape[0] + 1, dtype=np.int64)
+        rowptr[1:] = np.cumsum(counts)
+        cols = np.zeros(max(int(rowptr[-1]), 1), dtype=np.int32)
+        bpc = ctypes.c_int64(-1)
+        st = lib().orc_precomputed_lists(self.h, _ptr(x), _ptr(y), x.shape[0], int(max_neighbors),
+                                         _ptr(rowptr), _ptr(cols), ctypes.byref(bpc))
+        return st, bpc.value, (rowptr, cols[: rowptr[-1]])
+
+    def threaded_count_f32(self, y_nd, nthreads):
+        y = self._pts(y_nd)
+        counts = np.zeros(y.shape[0], dtype=np.int32)
+        tb, tt = ctypes.c_double(), ctypes.c_double()
+        st = lib().orc_threaded_count_f32(self.h, _ptr(y), y.shape[0], int(nthreads), _ptr(counts),
+                                          ctypes.byref(tb), ctypes.byref(tt))
+        return st, counts, tb.value, tt.value
+
+
+def trivial(x_nd, y_nd, radius, coord_dtype=np.float64, radius_dtype=np.float64, lists=False):
+    """TrivialNeighborhoodSearch brute force: counts and (optionally) lists (ascending)."""
+    x = np.ascontiguousarray(x_nd, dtype=coord_dtype)
+    y = np.ascontiguousarray(y_nd, dtype=coord_dtype)
+    nd = x.shape[1]
+    counts = np.zeros(x.shape[0], dtype=np.int32)
+    lib().orc_trivial_count(nd, code(coord_dtype), code(radius_dtype), _ptr(x), _ptr(y), x.shape[0],
+                            y.shape[0], float(radius), _ptr(counts), None, None)
+    if not lists:
+        return counts, None
+    rowptr = np.zeros(x.shape[0] + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(counts)
+    cols = np.zeros(max(int(rowptr[-1]), 1), dtype=np.int32)
+    lib().orc_trivial_count(nd, code(coord_dtype), code(radius_dtype), _ptr(x), _ptr(y), x.shape[0],
+                            y.shape[0], float(radius), None, _ptr(rowptr), _ptr(cols))
+    return counts, (rowptr, cols[: rowptr[-1]])
+
+
+def candidates_dict(y_nd, xq, cell_size):
+    y = np.ascontiguousarray(y_nd, dtype=np.float64)
+    q = np.ascontiguousarray(xq, dtype=np.float64)
+    out = np.zeros(y.shape[0], dtype=np.int64)
+    m = lib().orc_candidates_dict(y.shape[1], _ptr(y), y.shape[0], _ptr(q), float(cell_size), _ptr(out))
+    return out[:m].tolist()
+
+
+def cell_coords(x, min_corner, cell_size, coord_dtype=np.float64, cell_dtype=np.float64):
+    xa = np.ascontiguousarray(x, dtype=coord_dtype)
+    mc = np.ascontiguousarray(min_corner, dtype=np.float64)
+    out = np.zeros(3, dtype=np.int64)
+    lib().orc_cell_coords(len(xa), code(coord_dtype), code(cell_dtype), _ptr(xa), _ptr(mc), float(cell_size),
+                          _ptr(out))
+    return tuple(int(v) for v in out[: len(xa)])
+
+
+def sorted_lists(rowptr, cols):
+    """Sort each CSR row (for comparing traversal-order lists as sets)."""
+    rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
+    order = np.lexsort((cols, rows))
+    return cols[order]

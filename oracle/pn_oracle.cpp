@@ -1,0 +1,526 @@
+// Oracle: a CPU restatement of PointNeighbors.jl v0.6.7 for the
+// GridNeighborhoodSearch + FullGridCellList + PrecomputedNeighborhoodSearch path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+// pointneighbors.jl_amd/.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product path never does.
+//
+// Every function names the reference file:line it restates.  Arithmetic follows
+// Julia semantics exactly: IEEE single/double, no fused multiply-add (build with
+// -ffp-contract=off), left-to-right dot product (StaticArrays `dot` unrolled as
+// ((p1*p1 + p2*p2) + p3*p3)), `r^2` evaluated as r*r, IEEE sqrt and division.
+//
+// Pinning (what checks each function, see tests/test_oracle_golden.py):
+//   orc_floor_to_int*        test/nhs_grid.jl:62-86 saturation cases (fixture)
+//   orc_fullgrid_geometry    test/cell_lists/full_grid.jl:25-29 padded corners (fixture)
+//   orc_grid_build           test/nhs_grid.jl:197-225 FullGrid candidate list with
+//                            eachindex_y (fixture); bounds/overflow errors
+//                            test/cell_lists/full_grid.jl:31-64, test/vector_of_vectors.jl:53-58
+//   orc_candidates_dict      test/nhs_grid.jl:88-195 DictionaryCellList candidate lists
+//   orc_traverse_*           brute force (orc_trivial_*) on jittered clouds, as
+//                            test/neighborhood_search.jl:186-469 does; closed-form
+//                            lattice counts; test/gpu.jl:30-36 extrema (11, 29)
+//   orc_trivial_*            numpy brute force on small clouds (independent code)
+//   the d^2 evaluation order parity unpinned against the reference (StaticArrays is
+//                            not vendored); pinned to the stated reading by a numpy
+//                            step-rounded evaluation on adversarial pairs
+//   orc_sph_* / orc_energy_* parity unpinned against the reference (the physics lives in
+//                            TrixiParticles, absent); pinned by closed-form single-pair
+//                            values, symmetry, and forward-mode duals vs finite differences
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr int OK = 0;
+constexpr int ERR_ARG = 1;
+constexpr int ERR_OUT_OF_BOUNDS = 2;   // src/cell_lists/full_grid.jl:187-213
+constexpr int ERR_CELL_FULL = 3;       // src/vector_of_vectors.jl:114-121
+constexpr int ERR_QUERY_BOUNDS = 4;    // BoundsError from LinearIndices in traversal
+
+// src/util.jl:19-34.  Julia compares Float and Int exactly; 2^63 is exact in both
+// float and double, so `rounded >= typemax(Int)` is `rounded >= 2^63`.
+template <class T>
+int64_t floor_to_int(T x) {
+    T rounded = std::floor(x);
+    if (std::isnan(rounded) || rounded >= T(9223372036854775808.0)) return INT64_MAX;
+    if (rounded <= T(-9223372036854775808.0)) return INT64_MIN;
+    return (int64_t)rounded;
+}
+
+// Int64 `+ 1` wraps in Julia (test/nhs_grid.jl:83-85).
+inline int64_t wrap_add(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+
+struct Geom {
+    int ndims;
+    int64_t n[3];
+    double minc[3];   // padded min corner, exact in the cell type
+    double cs;        // cell size = search radius, exact in the cell type
+    int64_t ncells() const {
+        int64_t c = 1;
+        for (int d = 0; d < ndims; ++d) c *= n[d];
+        return c;
+    }
+};
+
+// src/cell_lists/full_grid.jl:84-94: floor_to_int.((x .- min_corner) ./ cell_size) .+ 1
+// The subtraction and division run in TCELL = promote(T_coords, T_min_corner).
+template <class TC, class TCELL>
+void cell_coords(const Geom& g, const TC* xp, int64_t cell[3]) {
+    for (int d = 0; d < g.ndims; ++d) {
+        TCELL q = ((TCELL)xp[d] - (TCELL)g.minc[d]) / (TCELL)g.cs;
+        cell[d] = wrap_add(floor_to_int<TCELL>(q), 1);
+    }
+}
+
+// src/cell_lists/full_grid.jl:187-213: every coordinate in 2:(n_d - 1)
+inline bool cell_in_bounds(const Geom& g, const int64_t cell[3]) {
+    for (int d = 0; d < g.ndims; ++d)
+        if (cell[d] < 2 || cell[d] > g.n[d] - 1) return false;
+    return true;
+}
+
+// LinearIndices (column-major, first dimension fastest), 0-based result.
+inline int64_t linear0(const Geom& g, const int64_t cell[3]) {
+    int64_t c = 0, stride = 1;
+    for (int d = 0; d < g.ndims; ++d) {
+        c += (cell[d] - 1) * stride;
+        stride *= g.n[d];
+    }
+    return c;
+}
+
+// pos_diff = convert.(T_r, x_i - x_j); d2 = dot(pos_diff, pos_diff)  (src/nhs_grid.jl:547-549)
+template <class TC, class TR>
+inline TR dist2(int ndims, const TC* xi, const TC* xj, TR pd[3]) {
+    for (int d = 0; d < ndims; ++d) pd[d] = (TR)(xi[d] - xj[d]);
+    TR d2 = pd[0] * pd[0];
+    for (int d = 1; d < ndims; ++d) {
+        TR p = pd[d] * pd[d];
+        d2 = d2 + p;
+    }
+    return d2;
+}
+
+// The DynamicVectorOfVectors column-per-cell storage (src/vector_of_vectors.jl:3-31).
+struct CellStore {
+    int64_t cap;                 // max_points_per_cell
+    std::vector<int32_t> lengths;
+    std::vector<int32_t> backend;  // cap x ncells, column per cell
+};
+
+struct Err {
+    int64_t point = -1;
+    int64_t cell[3] = {0, 0, 0};
+};
+
+// src/nhs_grid.jl:227-253 (serial build, insertion order) with the FullGrid push
+// src/cell_lists/full_grid.jl:112-121 and pushat! src/vector_of_vectors.jl:70-85.
+// The ParallelUpdate build (src/nhs_grid.jl:255-281) fills the same sets in a
+// nondeterministic order; sorted neighbour sets are identical.
+template <class TC, class TCELL>
+int build_serial(const Geom& g, const TC* y, const int64_t* active, int64_t n_active,
+                 CellStore& cs, Err& err) {
+    const int D = g.ndims;
+    cs.lengths.assign((size_t)g.ncells(), 0);
+    cs.backend.assign((size_t)(cs.cap * g.ncells()), 0);
+    for (int64_t a = 0; a < n_active; ++a) {
+        int64_t p = active[a];
+        int64_t cell[3];
+        cell_coords<TC, TCELL>(g, y + D * p, cell);
+        if (!cell_in_bounds(g, cell)) {
+            err.point = p;
+            for (int d = 0; d < D; ++d) err.cell[d] = cell[d];
+            return ERR_OUT_OF_BOUNDS;
+        }
+        int64_t c = linear0(g, cell);
+        int32_t len = ++cs.lengths[c];
+        if (len > cs.cap) {
+            --cs.lengths[c];
+            err.point = p;
+            return ERR_CELL_FULL;
+        }
+        cs.backend[(size_t)(c * cs.cap + (len - 1))] = (int32_t)p;
+    }
+    return OK;
+}
+
+// src/nhs_grid.jl:519-575 with neighboring_cells :577-583 (CartesianIndices,
+// first dimension fastest) and points_in_cell :593-597.  Calls f(j, pd, d2) for
+// every accepted neighbour in the reference's traversal order.
+template <class TC, class TR, class TCELL, class F>
+int traverse_point(const Geom& g, const CellStore& cs, const TC* y, const TC* xi, TR r, F&& f) {
+    const int D = g.ndims;
+    int64_t cell[3];
+    cell_coords<TC, TCELL>(g, xi, cell);
+    // LinearIndices getindex is bounds-checked in the safe variant: all 3^D neighbour
+    // cells must exist, i.e. the query cell lies in 2:(n_d - 1).
+    for (int d = 0; d < D; ++d)
+        if (cell[d] < 2 || cell[d] > g.n[d] - 1) return ERR_QUERY_BOUNDS;
+    const TR r2 = r * r;
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = d < D ? cell[d] - 1 : 1;
+        hi[d] = d < D ? cell[d] + 1 : 1;
+    }
+    for (int64_t c3 = lo[2]; c3 <= hi[2]; ++c3)
+        for (int64_t c2 = lo[1]; c2 <= hi[1]; ++c2)
+            for (int64_t c1 = lo[0]; c1 <= hi[0]; ++c1) {
+                int64_t nc[3] = {c1, c2, c3};
+                int64_t c = linear0(g, nc);
+                const int32_t* col = cs.backend.data() + c * cs.cap;
+                for (int32_t k = 0; k < cs.lengths[c]; ++k) {
+                    int32_t j = col[k];
+                    TR pd[3] = {0, 0, 0};
+                    TR d2 = dist2<TC, TR>(D, xi, y + D * (int64_t)j, pd);
+                    if (d2 <= r2) f(j, pd, d2);
+                }
+            }
+    return OK;
+}
+
+Geom make_geom(int ndims, const double* minc, double cs, const int64_t* n) {
+    Geom g;
+    g.ndims = ndims;
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = d < ndims ? n[d] : 1;
+        g.minc[d] = d < ndims ? minc[d] : 0.0;
+    }
+    g.cs = cs;
+    return g;
+}
+
+// One NHS instance kept alive across calls so that build and traversal are separate,
+// like initialize!/update! and foreach_point_neighbor.
+struct OracleGrid {
+    Geom g;
+    int tc, tr, tcell;   // 0 = Float32, 1 = Float64
+    CellStore cs;
+    double radius;
+    bool built = false;
+};
+
+template <class TC, class TR, class TCELL>
+struct Impl {
+    static int build(OracleGrid& og, const void* y, const int64_t* active, int64_t n_active,
+                     int64_t* err_point, int64_t* err_cell) {
+        Err e;
+        int st = build_serial<TC, TCELL>(og.g, (const TC*)y, active, n_active, og.cs, e);
+        if (err_point) *err_point = e.point;
+        if (err_cell)
+            for (int d = 0; d < og.g.ndims; ++d) err_cell[d] = e.cell[d];
+        og.built = (st == OK);
+        return st;
+    }
+
+    // counts[i] (for i in points) and, if rowptr != null, neighbour lists in
+    // traversal order.  The two-pass form mirrors count-then-fill.
+    static int traverse(const OracleGrid& og, const void* xv, const void* yv,
+                        const int64_t* points, int64_t n_points, int32_t* counts,
+                        int64_t* index_sums, const int64_t* rowptr, int32_t* cols,
+                        int64_t* err_point) {
+        const TC* x = (const TC*)xv;
+        const TC* y = (const TC*)yv;
+        const int D = og.g.ndims;
+        const TR r = (TR)og.radius;
+        for (int64_t a = 0; a < n_points; ++a) {
+            int64_t i = points[a];
+            int32_t cnt = 0;
+            int64_t isum = 0;
+            int64_t pos = rowptr ? rowptr[a] : 0;
+            int st = traverse_point<TC, TR, TCELL>(og.g, og.cs, y, x + D * i, r,
+                                                   [&](int32_t j, const TR*, TR) {
+                                                       ++cnt;
+                                                       isum += j;
+                                                       if (rowptr) cols[pos++] = j;
+                                                   });
+            if (st != OK) {
+                if (err_point) *err_point = i;
+                return st;
+            }
+            if (counts) counts[a] = cnt;
+            if (index_sums) index_sums[a] = isum;
+        }
+        return OK;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_floor_to_int_f64(double x) { return floor_to_int<double>(x); }
+int64_t orc_floor_to_int_f32(float x) { return floor_to_int<float>(x); }
+
+// FullGridCellList constructor geometry, src/cell_lists/full_grid.jl:48-82.
+//   user_dtype: element type of the user's min/max corner (0 Float32, 1 Float64)
+//   r_dtype:    type of search_radius
+// The padding `1001 // 1000 * search_radius` evaluates in T_r
+// (Rational{Int} * Float promotes to the float type, which converts 1001//1000 as
+// Float(1001)/Float(1000)); `min_corner .- pad` evaluates in promote(T_user, T_r).
+// n_cells = ceil.(Int, (max_corner .- min_corner) ./ search_radius).
+// Returns the corner type (0/1) or -1 for a template (search_radius < eps()).
+int orc_fullgrid_geometry(int ndims, int user_dtype, int r_dtype, const double* user_min,
+                          const double* user_max, double radius, double* out_min,
+                          double* out_max, int64_t* out_n) {
+    int tm = (user_dtype == 1 || r_dtype == 1) ? 1 : 0;
+    for (int d = 0; d < ndims; ++d) {
+        if (r_dtype == 0) {
+            float pad = (1001.0f / 1000.0f) * (float)radius;
+            if (tm == 0) {
+                out_min[d] = (double)((float)user_min[d] - pad);
+                out_max[d] = (double)((float)user_max[d] + pad);
+            } else {
+                out_min[d] = user_min[d] - (double)pad;
+                out_max[d] = user_max[d] + (double)pad;
+            }
+        } else {
+            double pad = (1001.0 / 1000.0) * radius;
+            out_min[d] = user_min[d] - pad;
+            out_max[d] = user_max[d] + pad;
+        }
+    }
+    if (radius < 2.220446049250313e-16) return -1;
+    for (int d = 0; d < ndims; ++d) {
+        if (tm == 0) {
+            float q = ((float)out_max[d] - (float)out_min[d]) / (float)radius;
+            out_n[d] = (int64_t)std::ceil(q);
+        } else {
+            double q = (out_max[d] - out_min[d]) / radius;
+            out_n[d] = (int64_t)std::ceil(q);
+        }
+    }
+    return tm;
+}
+
+// Cell coordinates of one point, src/cell_lists/full_grid.jl:84-94 (1-based, Int64 wrap).
+void orc_cell_coords(int ndims, int tc, int tcell, const void* x, const double* minc,
+                     double cs, int64_t* out) {
+    const int64_t ones[3] = {1, 1, 1};
+    Geom g = make_geom(ndims, minc, cs, ones);
+    if (tc == 0 && tcell == 0) cell_coords<float, float>(g, (const float*)x, out);
+    else if (tc == 0) cell_coords<float, double>(g, (const float*)x, out);
+    else cell_coords<double, double>(g, (const double*)x, out);
+}
+
+// DictionaryCellList cell coordinates, src/nhs_grid.jl:636-638: floor_to_int.(x ./ cs).
+// Used only to pin the floor-based cell indexing against test/nhs_grid.jl:88-195.
+// Writes the sorted candidate list (points of the 3^D cells around `xq`) to out and
+// returns its length.
+int64_t orc_candidates_dict(int ndims, const double* y, int64_t n, const double* xq,
+                            double cs, int64_t* out) {
+    std::vector<std::vector<int64_t>> cellc((size_t)n, std::vector<int64_t>(3, 0));
+    for (int64_t j = 0; j < n; ++j)
+        for (int d = 0; d < ndims; ++d) cellc[j][d] = floor_to_int<double>(y[ndims * j + d] / cs);
+    int64_t qc[3];
+    for (int d = 0; d < ndims; ++d) qc[d] = floor_to_int<double>(xq[d] / cs);
+    int64_t m = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        bool ok = true;
+        for (int d = 0; d < ndims; ++d)
+            if (cellc[j][d] < qc[d] - 1 || cellc[j][d] > qc[d] + 1) ok = false;
+        if (ok) out[m++] = j;
+    }
+    return m;
+}
+
+void* orc_grid_create(int ndims, int tc, int tr, int tcell, const double* minc,
+                      const int64_t* n, double radius, int64_t max_points_per_cell) {
+    OracleGrid* og = new OracleGrid();
+    og->g = make_geom(ndims, minc, radius, n);
+    og->tc = tc;
+    og->tr = tr;
+    og->tcell = tcell;
+    og->radius = radius;
+    og->cs.cap = max_points_per_cell;
+    return og;
+}
+
+void orc_grid_destroy(void* h) { delete (OracleGrid*)h; }
+
+#define ORC_DISPATCH(og, CALL)                                                        \
+    do {                                                                              \
+        if (og.tc == 0 && og.tr == 0 && og.tcell == 0) return Impl<float, float, float>::CALL; \
+        if (og.tc == 0 && og.tr == 0) return Impl<float, float, double>::CALL;       \
+        if (og.tc == 0 && og.tr == 1) return Impl<float, double, double>::CALL;      \
+        if (og.tc == 1 && og.tr == 0) return Impl<double, float, double>::CALL;      \
+        return Impl<double, double, double>::CALL;                                    \
+    } while (0)
+
+// initialize!/update! with ParallelUpdate semantics (src/nhs_grid.jl:220-292, :470-477).
+int orc_grid_build(void* h, const void* y, const int64_t* active, int64_t n_active,
+                   int64_t* err_point, int64_t* err_cell) {
+    OracleGrid& og = *(OracleGrid*)h;
+    if (og.radius < 2.220446049250313e-16) {  // src/nhs_grid.jl:263-267
+        og.cs.lengths.assign((size_t)og.g.ncells(), 0);
+        og.built = true;
+        return OK;
+    }
+    ORC_DISPATCH(og, build(og, y, active, n_active, err_point, err_cell));
+}
+
+// Lengths of each cell list after a build (DVoV `lengths`).
+void orc_grid_lengths(void* h, int32_t* out) {
+    OracleGrid& og = *(OracleGrid*)h;
+    std::memcpy(out, og.cs.lengths.data(), og.cs.lengths.size() * sizeof(int32_t));
+}
+
+// Points of one cell (0-based linear index) in insertion order; returns the length.
+int32_t orc_grid_cell(void* h, int64_t c, int32_t* out) {
+    OracleGrid& og = *(OracleGrid*)h;
+    int32_t len = og.cs.lengths[c];
+    std::memcpy(out, og.cs.backend.data() + c * og.cs.cap, (size_t)len * sizeof(int32_t));
+    return len;
+}
+
+// foreach_point_neighbor counting and sum-of-indices (the closures of
+// benchmarks/count_neighbors.jl:24-27 and test/neighborhood_search.jl:389-399),
+// optionally writing neighbour lists in traversal order.
+int orc_traverse(void* h, const void* x, const void* y, const int64_t* points,
+                 int64_t n_points, int32_t* counts, int64_t* index_sums, const int64_t* rowptr,
+                 int32_t* cols, int64_t* err_point) {
+    OracleGrid& og = *(OracleGrid*)h;
+    ORC_DISPATCH(og, traverse(og, x, y, points, n_points, counts, index_sums, rowptr, cols,
+                              err_point));
+}
+
+// TrivialNeighborhoodSearch (src/nhs_trivial.jl:55 with the generic loop
+// src/neighborhood_search.jl:390-421): every y point is a candidate.
+int orc_trivial_count(int ndims, int tc, int tr, const void* xv, const void* yv, int64_t nx,
+                      int64_t ny, double radius, int32_t* counts, const int64_t* rowptr,
+                      int32_t* cols) {
+    auto run = [&](auto tcv, auto trv) {
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        const TC* x = (const TC*)xv;
+        const TC* y = (const TC*)yv;
+        const TR r = (TR)radius;
+        const TR r2 = r * r;
+        for (int64_t i = 0; i < nx; ++i) {
+            int32_t cnt = 0;
+            int64_t pos = rowptr ? rowptr[i] : 0;
+            for (int64_t j = 0; j < ny; ++j) {
+                TR pd[3] = {0, 0, 0};
+                TR d2 = dist2<TC, TR>(ndims, x + ndims * i, y + ndims * j, pd);
+                if (d2 <= r2) {
+                    ++cnt;
+                    if (rowptr) cols[pos++] = (int32_t)j;
+                }
+            }
+            if (counts) counts[i] = cnt;
+        }
+    };
+    if (tc == 0 && tr == 0) run(0.0f, 0.0f);
+    else if (tc == 0) run(0.0f, 0.0);
+    else if (tr == 0) run(0.0, 0.0f);
+    else run(0.0, 0.0);
+    return OK;
+}
+
+// PrecomputedNeighborhoodSearch list build, src/nhs_precomputed.jl:187-206:
+// lists filled in traversal order, overflow error from pushat!
+// (src/vector_of_vectors.jl:79,114-121) when a list exceeds max_neighbors, then
+// sorteach! (src/vector_of_vectors.jl:177-184).  rowptr must come from a count pass.
+int orc_precomputed_lists(void* h, const void* x, const void* y, int64_t nx,
+                          int64_t max_neighbors, const int64_t* rowptr, int32_t* cols,
+                          int64_t* err_point) {
+    OracleGrid& og = *(OracleGrid*)h;
+    std::vector<int64_t> pts((size_t)nx);
+    for (int64_t i = 0; i < nx; ++i) pts[i] = i;
+    std::vector<int32_t> counts((size_t)nx);
+    int st = orc_traverse(h, x, y, pts.data(), nx, counts.data(), nullptr, rowptr, cols, err_point);
+    if (st != OK) return st;
+    for (int64_t i = 0; i < nx; ++i) {
+        if (counts[i] > max_neighbors) {
+            if (err_point) *err_point = i;
+            return ERR_CELL_FULL;
+        }
+        std::sort(cols + rowptr[i], cols + rowptr[i + 1]);
+    }
+    (void)og;
+    return OK;
+}
+
+// ---------------------------------------------------------------------------
+// Threaded CPU path with the reference's CPU semantics, used as the timed CPU
+// baseline: ParallelUpdate build with atomic fetch-add pushes
+// (src/nhs_grid.jl:255-281, src/vector_of_vectors.jl:93-112) and point-major
+// traversal in original index order, statically chunked over points like
+// Polyester.@batch (src/util.jl:133-137).  Float32 coordinates and radius, 3D.
+// Returns seconds for build and traversal through the out pointers.
+// ---------------------------------------------------------------------------
+int orc_threaded_count_f32(void* h, const float* y, int64_t n, int nthreads, int32_t* counts,
+                           double* t_build, double* t_trav) {
+    OracleGrid& og = *(OracleGrid*)h;
+    if (og.tc != 0 || og.tr != 0) return ERR_ARG;
+    const Geom& g = og.g;
+    const int D = g.ndims;
+    const int64_t C = g.ncells();
+    std::vector<std::atomic<int32_t>> lengths((size_t)C);
+    og.cs.backend.assign((size_t)(og.cs.cap * C), 0);
+    std::atomic<int> status{OK};
+    auto now = [] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+            .count();
+    };
+    auto chunked = [&](int64_t count, auto&& body) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; ++t) {
+            int64_t b = count * t / nthreads, e = count * (t + 1) / nthreads;
+            th.emplace_back([=, &body] {
+                for (int64_t i = b; i < e; ++i) body(i);
+            });
+        }
+        for (auto& x : th) x.join();
+    };
+    double t0 = now();
+    chunked(C, [&](int64_t c) { lengths[c].store(0, std::memory_order_relaxed); });
+    chunked(n, [&](int64_t p) {
+        int64_t cell[3];
+        if (og.tcell == 0) cell_coords<float, float>(g, y + D * p, cell);
+        else cell_coords<float, double>(g, y + D * p, cell);
+        if (!cell_in_bounds(g, cell)) {
+            status.store(ERR_OUT_OF_BOUNDS);
+            return;
+        }
+        int64_t c = linear0(g, cell);
+        int32_t len = lengths[c].fetch_add(1, std::memory_order_relaxed) + 1;
+        if (len > og.cs.cap) {
+            lengths[c].fetch_sub(1, std::memory_order_relaxed);
+            status.store(ERR_CELL_FULL);
+            return;
+        }
+        og.cs.backend[(size_t)(c * og.cs.cap + len - 1)] = (int32_t)p;
+    });
+    double t1 = now();
+    if (status.load() != OK) return status.load();
+    og.cs.lengths.resize((size_t)C);
+    for (int64_t c = 0; c < C; ++c) og.cs.lengths[c] = lengths[c].load();
+    og.built = true;
+    const float r = (float)og.radius;
+    std::atomic<int> qstatus{OK};
+    double t2 = now();
+    chunked(n, [&](int64_t i) {
+        int32_t cnt = 0;
+        int st = og.tcell == 0
+                     ? traverse_point<float, float, float>(g, og.cs, y, y + D * i, r,
+                                                           [&](int32_t, const float*, float) { ++cnt; })
+                     : traverse_point<float, float, double>(g, og.cs, y, y + D * i, r,
+                                                            [&](int32_t, const float*, float) { ++cnt; });
+        if (st != OK) qstatus.store(st);
+        counts[i] = cnt;
+    });
+    double t3 = now();
+    if (t_build) *t_build = t1 - t0;
+    if (t_trav) *t_trav = t3 - t2;
+    return qstatus.load();
+}
+
+}  // extern "C"

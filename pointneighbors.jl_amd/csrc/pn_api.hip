@@ -1,0 +1,620 @@
+// C-ABI of the MI355X neighbourhood-search engine: object lifetime, device buffers,
+// kernel launches and error reporting.  Declarations and the reference interfaces
+// each entry point replaces: include/pointneighbors_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/pointneighbors_hip.h"
+#include "pn_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PN_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(PN_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PN_CHECK_LAUNCH() PN_HIP(hipGetLastError())
+
+size_t dtype_size(int t) { return t == PN_FLOAT64 ? 8 : 4; }
+
+int64_t round_up8(int64_t n) { return (n + 7) / 8 * 8; }
+
+// Device buffer that only grows.
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need, hipStream_t stream) {
+        if (need <= bytes) return PN_OK;
+        if (p) {
+            PN_HIP(hipStreamSynchronize(stream));
+            PN_HIP(hipFree(p));
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t alloc = std::max<size_t>(need, 256);
+        PN_HIP(hipMalloc(&p, alloc));
+        bytes = alloc;
+        return PN_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// Exclusive scan over n int32 values (cell histograms and CSR row counts).
+struct Scanner {
+    Buf block_sums;
+    int run(const int32_t* counts, int64_t n, int32_t* start, int32_t* nonempty, pn::DevStatus* st,
+            hipStream_t stream) {
+        const int64_t nb = std::max<int64_t>(1, (n + pn::SCAN_TILE - 1) / pn::SCAN_TILE);
+        int rc = block_sums.ensure((size_t)nb * 8, stream);
+        if (rc) return rc;
+        auto* bs = block_sums.as<unsigned long long>();
+        hipLaunchKernelGGL(pn::k_scan_reduce, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts,
+                           n, bs, st);
+        PN_CHECK_LAUNCH();
+        hipLaunchKernelGGL(pn::k_scan_top, dim3(1), dim3(pn::SCAN_THREADS), 0, stream, bs, nb, start, n, st);
+        PN_CHECK_LAUNCH();
+        hipLaunchKernelGGL(pn::k_scan_down, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts, n,
+                           (const unsigned long long*)bs, start, nonempty);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    }
+};
+
+// One binned point set: the grid's y or a query set x.
+struct BinSet {
+    Buf counts, start, nonempty, keys, slots, perm_tmp, sorted_idx, coords, status;
+    pn::DevStatus* host_status = nullptr;  // pinned
+    Scanner scan;
+    int64_t cap = 0;
+    int64_t n_sel = 0;        // selected points of the last binning
+    int64_t n_src = 0;        // columns of the source matrix
+    const void* src = nullptr;
+    bool valid = false;       // last binning succeeded
+    bool stats_known = false;
+    pn::DevStatus stats{};
+
+    ~BinSet() {
+        counts.release();
+        start.release();
+        nonempty.release();
+        keys.release();
+        slots.release();
+        perm_tmp.release();
+        sorted_idx.release();
+        coords.release();
+        status.release();
+        scan.block_sums.release();
+        if (host_status) (void)hipHostFree(host_status);
+    }
+
+    pn::SetView view() const {
+        pn::SetView v;
+        v.start = start.as<int32_t>();
+        v.nonempty = nonempty.as<int32_t>();
+        v.sorted_idx = sorted_idx.as<int32_t>();
+        v.coords = coords.p;
+        v.cap = cap;
+        v.st = status.as<pn::DevStatus>();
+        return v;
+    }
+};
+
+}  // namespace
+
+struct pn_grid {
+    int nd = 0, tc = 0, tr = 0, tcell = 0;
+    pn::Geom geom{};
+    double radius = 0.0;
+    int32_t max_ppc = 100;
+    hipStream_t stream = nullptr;
+    BinSet grid;
+    BinSet query;
+    Buf csr_counts;
+    Buf csr_status;
+    Scanner csr_scan;
+    pn::DevStatus* csr_host_status = nullptr;
+    int64_t err_point = -1;
+    int64_t err_cell[3] = {0, 0, 0};
+    const void* err_src = nullptr;
+
+    ~pn_grid() {
+        csr_counts.release();
+        csr_status.release();
+        csr_scan.block_sums.release();
+        if (csr_host_status) (void)hipHostFree(csr_host_status);
+    }
+};
+
+namespace {
+
+// Template dispatch over (ndims, coordinate type, radius type, cell type).
+template <class Fn>
+int dispatch_build(const pn_grid& g, Fn&& fn) {
+    auto by_d = [&](auto tcv, auto tcellv) -> int {
+        using TC = decltype(tcv);
+        using TCELL = decltype(tcellv);
+        switch (g.nd) {
+            case 1: return fn(std::integral_constant<int, 1>{}, TC{}, TCELL{});
+            case 2: return fn(std::integral_constant<int, 2>{}, TC{}, TCELL{});
+            case 3: return fn(std::integral_constant<int, 3>{}, TC{}, TCELL{});
+        }
+        return fail(PN_ERR_ARGUMENT, "ndims must be 1, 2 or 3");
+    };
+    if (g.tc == PN_FLOAT32 && g.tcell == PN_FLOAT32) return by_d(float{}, float{});
+    if (g.tc == PN_FLOAT32) return by_d(float{}, double{});
+    return by_d(double{}, double{});
+}
+
+template <class Fn>
+int dispatch_trav(const pn_grid& g, Fn&& fn) {
+    auto by_d = [&](auto tcv, auto trv) -> int {
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        switch (g.nd) {
+            case 1: return fn(std::integral_constant<int, 1>{}, TC{}, TR{});
+            case 2: return fn(std::integral_constant<int, 2>{}, TC{}, TR{});
+            case 3: return fn(std::integral_constant<int, 3>{}, TC{}, TR{});
+        }
+        return fail(PN_ERR_ARGUMENT, "ndims must be 1, 2 or 3");
+    };
+    if (g.tc == PN_FLOAT32 && g.tr == PN_FLOAT32) return by_d(float{}, float{});
+    if (g.tc == PN_FLOAT32) return by_d(float{}, double{});
+    if (g.tr == PN_FLOAT32) return by_d(double{}, float{});
+    return by_d(double{}, double{});
+}
+
+int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
+    const int64_t C = g.geom.ncells;
+    int rc;
+    if ((rc = s.counts.ensure((size_t)C * 4, g.stream))) return rc;
+    if ((rc = s.start.ensure((size_t)(C + 1) * 4, g.stream))) return rc;
+    if ((rc = s.nonempty.ensure((size_t)C * 4, g.stream))) return rc;
+    if ((rc = s.status.ensure(sizeof(pn::DevStatus), g.stream))) return rc;
+    if (!s.host_status) PN_HIP(hipHostMalloc((void**)&s.host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
+    if (n_sel > s.cap) {
+        int64_t cap = std::max<int64_t>(n_sel, 64);
+        if ((rc = s.keys.ensure((size_t)cap * 4, g.stream))) return rc;
+        if ((rc = s.slots.ensure((size_t)cap * 4, g.stream))) return rc;
+        if ((rc = s.perm_tmp.ensure((size_t)cap * 4, g.stream))) return rc;
+        if ((rc = s.sorted_idx.ensure((size_t)cap * 4, g.stream))) return rc;
+        // planes are strided by cap: reallocate exactly
+        PN_HIP(hipStreamSynchronize(g.stream));
+        s.coords.release();
+        if ((rc = s.coords.ensure((size_t)cap * g.nd * dtype_size(g.tc), g.stream))) return rc;
+        s.cap = cap;
+    }
+    return PN_OK;
+}
+
+unsigned blocks_for(int64_t n, int threads) { return (unsigned)std::max<int64_t>(1, (n + threads - 1) / threads); }
+
+// Enqueues the counting-sort binning of the selected points of src into s.
+int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t sel_first, int64_t sel_count,
+                const int32_t* sel_list) {
+    if (!src && sel_count > 0) return fail(PN_ERR_ARGUMENT, "coordinate pointer is NULL");
+    if (sel_count < 0 || n_src < 0) return fail(PN_ERR_ARGUMENT, "negative size");
+    if (n_src > INT32_MAX - 1 || sel_count > INT32_MAX - 1)
+        return fail(PN_ERR_ARGUMENT, "more than 2^31 - 2 points are not supported");
+    if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_src))
+        return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the coordinate matrix");
+    int rc = ensure_set(g, s, sel_count);
+    if (rc) return rc;
+    const int64_t C = g.geom.ncells;
+    auto* st = s.status.as<pn::DevStatus>();
+    hipLaunchKernelGGL(pn::k_reset_status, dim3(1), dim3(1), 0, g.stream, st);
+    PN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(C, 1 << 20), 256)), dim3(256), 0,
+                       g.stream, s.counts.as<int32_t>(), C, 0);
+    PN_CHECK_LAUNCH();
+    rc = dispatch_build(g, [&](auto dc, auto tcv, auto tcellv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TCELL = decltype(tcellv);
+        if (sel_count > 0) {
+            hipLaunchKernelGGL((pn::k_bin<D, TC, TCELL>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
+                               g.geom, (const TC*)src, sel_first, sel_list, sel_count, s.keys.as<int32_t>(),
+                               s.slots.as<int32_t>(), s.counts.as<int32_t>(), st);
+            PN_CHECK_LAUNCH();
+        }
+        int rc2 = s.scan.run(s.counts.as<int32_t>(), C, s.start.as<int32_t>(), s.nonempty.as<int32_t>(), st,
+                             g.stream);
+        if (rc2) return rc2;
+        if (sel_count > 0) {
+            hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
+                               s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_first, sel_list, sel_count,
+                               s.start.as<int32_t>(), s.perm_tmp.as<int32_t>());
+            PN_CHECK_LAUNCH();
+            hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
+                               s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_first, sel_list, sel_count,
+                               s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
+                               s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap);
+            PN_CHECK_LAUNCH();
+        }
+        return PN_OK;
+    });
+    if (rc) return rc;
+    s.n_sel = sel_count;
+    s.n_src = n_src;
+    s.src = src;
+    s.valid = true;  // provisional until checked
+    s.stats_known = false;
+    return PN_OK;
+}
+
+// Host coordinates and 1-based cell of one point (error path only).
+int describe_point(pn_grid& g, const void* src, int64_t p) {
+    g.err_point = p;
+    double xs[3] = {0, 0, 0};
+    if (g.tc == PN_FLOAT32) {
+        float f[3];
+        PN_HIP(hipMemcpy(f, (const float*)src + g.nd * p, g.nd * 4, hipMemcpyDeviceToHost));
+        for (int d = 0; d < g.nd; ++d) xs[d] = f[d];
+    } else {
+        PN_HIP(hipMemcpy(xs, (const double*)src + g.nd * p, g.nd * 8, hipMemcpyDeviceToHost));
+    }
+    for (int d = 0; d < g.nd; ++d) {
+        double q;
+        if (g.tcell == PN_FLOAT32)
+            q = (double)(((float)xs[d] - (float)g.geom.minc[d]) / (float)g.geom.cs);
+        else
+            q = (xs[d] - g.geom.minc[d]) / g.geom.cs;
+        double r = std::floor(q);
+        int64_t c;
+        if (std::isnan(r) || r >= 9223372036854775808.0) c = INT64_MAX;
+        else if (r <= -9223372036854775808.0) c = INT64_MIN;
+        else c = (int64_t)r;
+        g.err_cell[d] = (int64_t)((uint64_t)c + 1u);
+    }
+    return PN_OK;
+}
+
+// Synchronizes and turns the device status of a binning pass into the reference's errors.
+int check_set(pn_grid& g, BinSet& s, const int32_t* sel_list, int64_t sel_first, bool is_query) {
+    PN_HIP(hipMemcpyAsync(s.host_status, s.status.p, sizeof(pn::DevStatus), hipMemcpyDeviceToHost, g.stream));
+    PN_HIP(hipStreamSynchronize(g.stream));
+    s.stats = *s.host_status;
+    s.stats_known = true;
+    if (s.stats.first_bad != ~0ull) {
+        s.valid = false;
+        int64_t a = (int64_t)s.stats.first_bad;
+        int64_t p = a + sel_first;
+        if (sel_list) {
+            int32_t pi;
+            PN_HIP(hipMemcpy(&pi, sel_list + a, 4, hipMemcpyDeviceToHost));
+            p = pi;
+        }
+        int rc = describe_point(g, s.src, p);
+        if (rc) return rc;
+        if (is_query)
+            return fail(PN_ERR_QUERY_BOUNDS,
+                        "BoundsError: the neighbor cells of a query point are out of bounds of the cell grid");
+        return fail(PN_ERR_OUT_OF_BOUNDS,
+                    "particle coordinates are NaN or outside the domain bounds of the cell list");
+    }
+    if (!is_query && s.stats.max_count > g.max_ppc) {
+        s.valid = false;
+        return fail(PN_ERR_CELL_FULL, "cell list is full. Use a larger `max_points_per_cell`.");
+    }
+    return PN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn_abi_version(void) { return PN_ABI_VERSION; }
+
+const char* pn_last_error(void) { return g_err.c_str(); }
+
+int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype, const double* min_corner,
+                   const int64_t* n_cells, double search_radius, int32_t max_points_per_cell, pn_grid_t* out) {
+    if (!out) return fail(PN_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (ndims < 1 || ndims > 3) return fail(PN_ERR_ARGUMENT, "ndims must be 1, 2 or 3");
+    for (int t : {coord_dtype, radius_dtype, cell_dtype})
+        if (t != PN_FLOAT32 && t != PN_FLOAT64) return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    if (cell_dtype == PN_FLOAT32 && (coord_dtype != PN_FLOAT32 || radius_dtype != PN_FLOAT32))
+        return fail(PN_ERR_ARGUMENT, "cell_dtype must be the promotion of the coordinate and corner types");
+    if (!(search_radius > 0.0) || !std::isfinite(search_radius))
+        return fail(PN_ERR_ARGUMENT, "search_radius must be positive and finite");
+    if (max_points_per_cell < 1) return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be positive");
+    int64_t C = 1;
+    for (int d = 0; d < ndims; ++d) {
+        if (n_cells[d] < 3) return fail(PN_ERR_ARGUMENT, "every dimension needs at least 3 cells");
+        if (n_cells[d] > INT32_MAX) return fail(PN_ERR_ARGUMENT, "too many cells");
+        C *= n_cells[d];
+        if (C > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "more than 2^31 - 2 cells are not supported");
+    }
+    pn_grid* g = new pn_grid();
+    g->nd = ndims;
+    g->tc = coord_dtype;
+    g->tr = radius_dtype;
+    g->tcell = cell_dtype;
+    g->radius = search_radius;
+    g->max_ppc = max_points_per_cell;
+    g->geom.nd = ndims;
+    for (int d = 0; d < 3; ++d) {
+        g->geom.n[d] = d < ndims ? (int)n_cells[d] : 1;
+        g->geom.minc[d] = d < ndims ? min_corner[d] : 0.0;
+    }
+    g->geom.ncells = (int)C;
+    g->geom.cs = search_radius;
+    *out = g;
+    return PN_OK;
+}
+
+int pn_grid_destroy(pn_grid_t grid) {
+    if (!grid) return PN_OK;
+    if (grid->stream) (void)hipStreamSynchronize(grid->stream);
+    else (void)hipDeviceSynchronize();
+    delete grid;
+    return PN_OK;
+}
+
+int pn_grid_set_stream(pn_grid_t grid, void* stream) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    grid->stream = (hipStream_t)stream;
+    return PN_OK;
+}
+
+int pn_grid_update_async(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first, int64_t sel_count,
+                         const int32_t* sel_list) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    grid->err_src = y;
+    return enqueue_bin(*grid, grid->grid, y, n_y, sel_first, sel_count, sel_list);
+}
+
+int pn_grid_check(pn_grid_t grid) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    return check_set(*grid, grid->grid, nullptr, 0, false);
+}
+
+int pn_grid_update(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first, int64_t sel_count,
+                   const int32_t* sel_list) {
+    int rc = pn_grid_update_async(grid, y, n_y, sel_first, sel_count, sel_list);
+    if (rc) {
+        if (grid) grid->grid.valid = false;
+        return rc;
+    }
+    return check_set(*grid, grid->grid, sel_list, sel_first, false);
+}
+
+int pn_grid_error(pn_grid_t grid, int64_t* point, int64_t* cell) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (point) *point = grid->err_point;
+    if (cell)
+        for (int d = 0; d < grid->nd; ++d) cell[d] = grid->err_cell[d];
+    return PN_OK;
+}
+
+int pn_grid_refresh(pn_grid_t grid, const void* y, int64_t n_y) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    BinSet& s = grid->grid;
+    if (!s.valid) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    if (n_y != s.n_src) return fail(PN_ERR_ARGUMENT, "coordinate matrix size differs from the last update!");
+    if (s.n_sel == 0) return PN_OK;
+    return dispatch_build(*grid, [&](auto dc, auto tcv, auto) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        hipLaunchKernelGGL((pn::k_regather<D, TC>), dim3(blocks_for(s.n_sel, 256)), dim3(256), 0, grid->stream,
+                           s.sorted_idx.as<int32_t>(), s.status.as<pn::DevStatus>(), (const TC*)y,
+                           s.coords.as<TC>(), s.cap);
+        PN_CHECK_LAUNCH();
+        s.src = y;
+        return PN_OK;
+    });
+}
+
+int pn_grid_bin_queries(pn_grid_t grid, const void* x, int64_t n_x, int64_t sel_first, int64_t sel_count,
+                        const int32_t* sel_list) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    int rc = enqueue_bin(*grid, grid->query, x, n_x, sel_first, sel_count, sel_list);
+    if (rc) {
+        grid->query.valid = false;
+        return rc;
+    }
+    return check_set(*grid, grid->query, sel_list, sel_first, true);
+}
+
+int pn_grid_stats(pn_grid_t grid, int64_t* n_binned, int64_t* n_nonempty, int64_t* max_count) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    BinSet& s = grid->grid;
+    if (!s.valid) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    if (!s.stats_known) {
+        PN_HIP(hipMemcpyAsync(s.host_status, s.status.p, sizeof(pn::DevStatus), hipMemcpyDeviceToHost,
+                              grid->stream));
+        PN_HIP(hipStreamSynchronize(grid->stream));
+        s.stats = *s.host_status;
+        s.stats_known = true;
+    }
+    if (n_binned) *n_binned = s.stats.n_binned;
+    if (n_nonempty) *n_nonempty = s.stats.n_nonempty;
+    if (max_count) *max_count = s.stats.max_count;
+    return PN_OK;
+}
+
+int pn_grid_copy_cells(pn_grid_t grid, int32_t* cell_start, int32_t* sorted_index) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    BinSet& s = grid->grid;
+    if (!s.valid) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    if (cell_start)
+        PN_HIP(hipMemcpyAsync(cell_start, s.start.p, (size_t)(grid->geom.ncells + 1) * 4, hipMemcpyDeviceToDevice,
+                              grid->stream));
+    if (sorted_index && s.n_sel > 0)
+        PN_HIP(hipMemcpyAsync(sorted_index, s.sorted_idx.p, (size_t)s.n_sel * 4, hipMemcpyDeviceToDevice,
+                              grid->stream));
+    return PN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+const BinSet* query_set(pn_grid& g, int queries) {
+    if (queries == PN_QUERIES_GRID) return g.grid.valid ? &g.grid : nullptr;
+    if (queries == PN_QUERIES_BINNED) return (g.query.valid && g.grid.valid) ? &g.query : nullptr;
+    return nullptr;
+}
+
+// Candidate chunk staged in LDS per wavefront.
+template <class TC>
+struct Chunk {
+    static constexpr int value = sizeof(TC) == 4 ? 512 : 256;
+};
+
+template <class F>
+int launch_traverse(pn_grid& g, int queries, const F& f) {
+    const BinSet* Q = query_set(g, queries);
+    if (!Q) {
+        if (queries != PN_QUERIES_GRID && queries != PN_QUERIES_BINNED)
+            return fail(PN_ERR_ARGUMENT, "queries must be PN_QUERIES_GRID or PN_QUERIES_BINNED");
+        return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    }
+    if (Q->n_sel == 0) return PN_OK;
+    // Upper bound on non-empty query cells; the kernel reads the exact count on device.
+    int64_t nwork = std::min<int64_t>(g.geom.ncells, Q->n_sel);
+    if (Q->stats_known) nwork = Q->stats.n_nonempty;
+    if (nwork == 0) return PN_OK;
+    const int nblocks = (int)round_up8(nwork);
+    return dispatch_trav(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        const TR r = (TR)g.radius;
+        const TR r2 = r * r;
+        hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC>::value>), dim3(nblocks), dim3(pn::WAVE), 0,
+                           g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn_count_neighbors(pn_grid_t grid, int queries, int32_t* counts, int accumulate) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!counts) return fail(PN_ERR_ARGUMENT, "counts is NULL");
+    pn::CountF f{counts, accumulate};
+    return launch_traverse(*grid, queries, f);
+}
+
+int pn_sum_neighbor_indices(pn_grid_t grid, int queries, int64_t* sums, int index_base, int accumulate) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!sums) return fail(PN_ERR_ARGUMENT, "sums is NULL");
+    pn::IndexSumF f{sums, accumulate, index_base};
+    return launch_traverse(*grid, queries, f);
+}
+
+int pn_csr_count(pn_grid_t grid, int queries, int64_t n_rows, int32_t* row_ptr, int32_t max_neighbors,
+                 int64_t* nnz, int64_t* max_row) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!row_ptr) return fail(PN_ERR_ARGUMENT, "row_ptr is NULL");
+    pn_grid& g = *grid;
+    const BinSet* Q = query_set(g, queries);
+    if (!Q) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    if (n_rows < Q->n_src) return fail(PN_ERR_ARGUMENT, "n_rows is smaller than the number of query points");
+    if (n_rows > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "too many rows");
+    int rc;
+    if ((rc = g.csr_counts.ensure((size_t)std::max<int64_t>(n_rows, 1) * 4, g.stream))) return rc;
+    if ((rc = g.csr_status.ensure(sizeof(pn::DevStatus), g.stream))) return rc;
+    if (!g.csr_host_status)
+        PN_HIP(hipHostMalloc((void**)&g.csr_host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
+    auto* st = g.csr_status.as<pn::DevStatus>();
+    hipLaunchKernelGGL(pn::k_reset_status, dim3(1), dim3(1), 0, g.stream, st);
+    PN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(n_rows, 1 << 20), 256)), dim3(256), 0,
+                       g.stream, g.csr_counts.as<int32_t>(), n_rows, 0);
+    PN_CHECK_LAUNCH();
+    pn::CountF f{g.csr_counts.as<int32_t>(), 0};
+    if ((rc = launch_traverse(g, queries, f))) return rc;
+    if ((rc = g.csr_scan.run(g.csr_counts.as<int32_t>(), n_rows, row_ptr, nullptr, st, g.stream))) return rc;
+    PN_HIP(hipMemcpyAsync(g.csr_host_status, st, sizeof(pn::DevStatus), hipMemcpyDeviceToHost, g.stream));
+    PN_HIP(hipStreamSynchronize(g.stream));
+    const pn::DevStatus hs = *g.csr_host_status;
+    if (nnz) *nnz = (int64_t)hs.total64;
+    if (max_row) *max_row = hs.max_count;
+    if (hs.total64 > (unsigned long long)INT32_MAX)
+        return fail(PN_ERR_UNSUPPORTED, "more than 2^31 - 1 neighbor pairs do not fit int32 row pointers");
+    if (max_neighbors > 0 && hs.max_count > max_neighbors)
+        return fail(PN_ERR_LIST_FULL, "cell list is full. Use a larger `max_points_per_cell`.");
+    return PN_OK;
+}
+
+int pn_csr_fill(pn_grid_t grid, int queries, int64_t n_rows, const int32_t* row_ptr, int32_t* col_idx,
+                int index_base) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!row_ptr) return fail(PN_ERR_ARGUMENT, "row_ptr is NULL");
+    pn_grid& g = *grid;
+    const BinSet* Q = query_set(g, queries);
+    if (!Q) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    if (n_rows < Q->n_src) return fail(PN_ERR_ARGUMENT, "n_rows is smaller than the number of query points");
+    if (g.csr_host_status && g.csr_host_status->max_count > pn::SORT_CAP)
+        return fail(PN_ERR_UNSUPPORTED, "neighbor lists longer than 4096 entries are not supported");
+    pn::FillF f{row_ptr, col_idx, index_base};
+    int rc = launch_traverse(g, queries, f);
+    if (rc) return rc;
+    if (n_rows == 0) return PN_OK;
+    const int nblocks = (int)round_up8(n_rows);
+    hipLaunchKernelGGL(pn::k_sort_rows, dim3(nblocks), dim3(pn::WAVE), 0, g.stream, row_ptr, n_rows, col_idx,
+                       nblocks);
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int pn_list_count_neighbors(const int32_t* row_ptr, int64_t n_rows, int64_t sel_first, int64_t sel_count,
+                            const int32_t* sel_list, int32_t* counts, int accumulate, void* stream) {
+    if (!row_ptr || !counts) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
+        return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
+    if (sel_count <= 0) return PN_OK;
+    hipLaunchKernelGGL(pn::k_list_count, dim3(blocks_for(sel_count, 256)), dim3(256), 0, (hipStream_t)stream,
+                       row_ptr, sel_first, sel_list, sel_count, counts, accumulate);
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+int pn_list_sum_neighbor_indices(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
+                                 int stored_base, int64_t sel_first, int64_t sel_count, const int32_t* sel_list,
+                                 int64_t* sums, int index_base, int accumulate, void* stream) {
+    if (!row_ptr || !sums) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
+        return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
+    if (sel_count <= 0) return PN_OK;
+    const int nblocks = (int)round_up8(sel_count);
+    hipLaunchKernelGGL(pn::k_list_index_sum, dim3(nblocks), dim3(pn::WAVE), 0, (hipStream_t)stream, row_ptr,
+                       col_idx, sel_first, sel_list, sel_count, (long long)index_base - stored_base, sums,
+                       accumulate, nblocks);
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+"""ctypes binding of libpointneighbors_hip.so (the C-ABI declared in
+include/pointneighbors_hip.h) and the mapping of its status codes to the reference's
+exceptions.
+
+The library is the only compute path: if it is missing or cannot be loaded, every call
+raises; there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpointneighbors_hip.so")
+
+PN_OK = 0
+PN_ERR_ARGUMENT = 1
+PN_ERR_OUT_OF_BOUNDS = 2
+PN_ERR_CELL_FULL = 3
+PN_ERR_QUERY_BOUNDS = 4
+PN_ERR_NOT_INITIALIZED = 5
+PN_ERR_HIP = 6
+PN_ERR_LIST_FULL = 7
+PN_ERR_UNSUPPORTED = 8
+
+PN_FLOAT32 = 0
+PN_FLOAT64 = 1
+
+PN_QUERIES_GRID = 0
+PN_QUERIES_BINNED = 1
+
+
+class ArgumentError(ValueError):
+    """Julia's `ArgumentError`."""
+
+
+class BoundsError(IndexError):
+    """Julia's `BoundsError`."""
+
+
+class PointNeighborsError(RuntimeError):
+    """Julia's `ErrorException` raised by `error(...)` in the reference."""
+
+
+_c_int = ctypes.c_int
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+_p_i64 = ctypes.POINTER(ctypes.c_int64)
+_p_dbl = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); must match include/pointneighbors_hip.h
+SIGNATURES = {
+    "pn_abi_version": (_c_int, []),
+    "pn_last_error": (ctypes.c_char_p, []),
+    "pn_grid_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _p_dbl, _p_i64, _c_dbl, _c_i32,
+                                ctypes.POINTER(_vp)]),
+    "pn_grid_destroy": (_c_int, [_vp]),
+    "pn_grid_set_stream": (_c_int, [_vp, _vp]),
+    "pn_grid_update": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_i64, _vp]),
+    "pn_grid_update_async": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_i64, _vp]),
+    "pn_grid_check": (_c_int, [_vp]),
+    "pn_grid_error": (_c_int, [_vp, _p_i64, _p_i64]),
+    "pn_grid_refresh": (_c_int, [_vp, _vp, _c_i64]),
+    "pn_grid_bin_queries": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_i64, _vp]),
+    "pn_grid_stats": (_c_int, [_vp, _p_i64, _p_i64, _p_i64]),
+    "pn_grid_copy_cells": (_c_int, [_vp, _vp, _vp]),
+    "pn_count_neighbors": (_c_int, [_vp, _c_int, _vp, _c_int]),
+    "pn_sum_neighbor_indices": (_c_int, [_vp, _c_int, _vp, _c_int, _c_int]),
+    "pn_csr_count": (_c_int, [_vp, _c_int, _c_i64, _vp, _c_i32, _p_i64, _p_i64]),
+    "pn_csr_fill": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp, _c_int]),
+    "pn_list_count_neighbors": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_int, _vp]),
+    "pn_list_sum_neighbor_indices": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_i64, _c_i64, _vp, _vp,
+                                              _c_int, _c_int, _vp]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the HIP library (once).  Raises OSError when it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(
+            f"{p} is missing: build it with `python pointneighbors.jl_amd/build.py` "
+            "(the MI355X path has no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.pn_abi_version() != 1:
+        raise OSError("ABI version mismatch of libpointneighbors_hip.so")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().pn_last_error().decode()
+
+
+def check(status: int, context: str = "") -> None:
+    """Raise the reference's exception type for a non-zero status."""
+    if status == PN_OK:
+        return
+    msg = last_error()
+    if context:
+        msg = f"{msg} ({context})" if msg else context
+    if status == PN_ERR_ARGUMENT:
+        raise ArgumentError(msg)
+    if status == PN_ERR_QUERY_BOUNDS:
+        raise BoundsError(msg)
+    if status in (PN_ERR_OUT_OF_BOUNDS, PN_ERR_CELL_FULL, PN_ERR_LIST_FULL,
+                  PN_ERR_NOT_INITIALIZED):
+        raise PointNeighborsError(msg)
+    raise RuntimeError(f"pointneighbors HIP error {status}: {msg}")

@@ -1,0 +1,327 @@
+"""GridNeighborhoodSearch with FullGridCellList (src/nhs_grid.jl) on the MI355X engine.
+
+initialize!/update! bin the points of y into cells with the device counting sort
+(`pn_grid_update`); foreach_point_neighbor runs the wave-per-cell traversal kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import itertools
+
+import numpy as np
+
+from . import _device
+from ._lib import (PN_ERR_OUT_OF_BOUNDS, PN_QUERIES_BINNED, PN_QUERIES_GRID, ArgumentError,
+                   BoundsError, PointNeighborsError, check, load)
+from .cell_lists import FullGridCellList
+from .util import FLOAT64_EPS, dtype_code, floor_to_int, promote, scalar_dtype, wrap_int64
+
+
+class _Strategy:
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self))
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+
+class ParallelUpdate(_Strategy):
+    """src/nhs_grid.jl:144.  On the MI355X engine: the counting-sort rebuild."""
+
+
+class ParallelIncrementalUpdate(_Strategy):
+    """src/nhs_grid.jl:156.  Accepted for drop-in; runs the same counting-sort rebuild
+    (identical neighbour sets)."""
+
+
+class SemiParallelUpdate(_Strategy):
+    """src/nhs_grid.jl:168.  Accepted for drop-in; runs the counting-sort rebuild."""
+
+
+class SerialIncrementalUpdate(_Strategy):
+    """src/nhs_grid.jl:180.  Accepted for drop-in; runs the counting-sort rebuild."""
+
+
+class SerialUpdate(_Strategy):
+    """src/nhs_grid.jl:192.  Accepted for drop-in; runs the counting-sort rebuild, whose
+    order inside a cell (original index ascending) equals the serial insertion order."""
+
+
+# src/cell_lists/full_grid.jl:39-42: ParallelUpdate first, so it is the default.
+SUPPORTED_UPDATE_STRATEGIES = (ParallelUpdate, ParallelIncrementalUpdate, SemiParallelUpdate,
+                               SerialIncrementalUpdate, SerialUpdate)
+
+
+class PeriodicBox:
+    """PeriodicBox(; min_corner, max_corner) (src/neighborhood_search.jl:129-141)."""
+
+    def __init__(self, *, min_corner, max_corner):
+        self.min_corner = np.asarray(min_corner, dtype=np.float64)
+        self.max_corner = np.asarray(max_corner, dtype=np.float64)
+        self.size = self.max_corner - self.min_corner
+
+
+class _GridMeta(type):
+    def __getitem__(cls, ndims):
+        return lambda **kw: cls(ndims, **kw)
+
+
+class GridNeighborhoodSearch(metaclass=_GridMeta):
+    """GridNeighborhoodSearch{NDIMS}(; search_radius = 0.0, n_points = 0,
+    periodic_box = nothing, cell_list, update_strategy = nothing)
+    (src/nhs_grid.jl:77-129).  Write `GridNeighborhoodSearch[3](...)` or
+    `GridNeighborhoodSearch(3, ...)`.
+
+    The MI355X engine requires a FullGridCellList (the reference's GPU-compatible cell
+    list); the DictionaryCellList default of the reference is a CPU hash table.
+    """
+
+    def __init__(self, ndims, *, search_radius=0.0, n_points=0, periodic_box=None,
+                 cell_list=None, update_strategy=None):
+        if cell_list is None:
+            raise ArgumentError(
+                "the MI355X engine needs `cell_list = FullGridCellList(...)`; "
+                "DictionaryCellList is a CPU hash table")
+        if not isinstance(cell_list, FullGridCellList):
+            raise ArgumentError(f"cell list {type(cell_list).__name__} is not supported")
+        if cell_list.ndims != ndims:
+            raise ArgumentError(f"a {ndims}D cell list is required for a GridNeighborhoodSearch{{{ndims}}}")
+        if update_strategy is None:
+            update_strategy = SUPPORTED_UPDATE_STRATEGIES[0]()
+        elif type(update_strategy) not in SUPPORTED_UPDATE_STRATEGIES:
+            names = ", ".join(t.__name__ for t in SUPPORTED_UPDATE_STRATEGIES)
+            raise ArgumentError(f"{update_strategy} is not a valid update strategy for this cell list. "
+                                f"Available options are ({names})")
+        r_dt = scalar_dtype(search_radius)
+        if r_dt.kind in "iub":
+            raise ArgumentError("`search_radius` cannot be an integer type, since computed "
+                                "distances will be converted to this type")
+        if periodic_box is not None:
+            raise ArgumentError("PeriodicBox is not supported by the MI355X engine yet")
+        self.ndims = ndims
+        self.search_radius = search_radius
+        self.radius_dtype = r_dt
+        self.n_points = n_points
+        self.periodic_box = periodic_box
+        self.cell_list = cell_list
+        self.update_strategy = update_strategy
+        self.cell_size = tuple(search_radius for _ in range(ndims))
+        self.n_cells = tuple(-1 for _ in range(ndims))
+        self._handle = None
+        self._coord_dtype = None
+        self._built = False
+        self._empty = False     # initialized with search_radius < eps()
+        self._y_key = None      # (data_ptr, shape, stride, version) of the last update
+        self._y_full = True     # eachindex_y covered all columns
+        self._refreshed = False
+        self._n_y = 0
+
+    # -- reference interface -------------------------------------------------------
+    def requires_update(self):
+        """src/nhs_grid.jl:133"""
+        return (False, True)
+
+    def eltype(self):
+        return self.radius_dtype
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None:
+            try:
+                load().pn_grid_destroy(h)
+            except Exception:
+                pass
+
+    # -- device handle -------------------------------------------------------------
+    def _ensure_handle(self, coord_dtype):
+        if self._handle is not None and self._coord_dtype == coord_dtype:
+            return
+        lib = load()
+        if self._handle is not None:
+            lib.pn_grid_destroy(self._handle)
+            self._handle = None
+        cl = self.cell_list
+        cell_dt = promote(coord_dtype, cl.corner_dtype)
+        nd = self.ndims
+        mins = (ctypes.c_double * nd)(*[float(v) for v in cl.min_corner])
+        ncs = (ctypes.c_int64 * nd)(*cl.n_cells)
+        h = ctypes.c_void_p()
+        check(lib.pn_grid_create(nd, dtype_code(coord_dtype), dtype_code(self.radius_dtype),
+                                 dtype_code(cell_dt), mins, ncs, float(self.search_radius),
+                                 int(cl.max_points_per_cell), ctypes.byref(h)),
+              "GridNeighborhoodSearch")
+        self._handle = h
+        self._coord_dtype = coord_dtype
+        self._cell_dtype = cell_dt
+
+    def _set_stream(self, t):
+        load().pn_grid_set_stream(self._handle, ctypes.c_void_p(_device.current_stream_handle(t.device)))
+
+    def host_cell_coords(self, coords):
+        """cell_coords (src/nhs_grid.jl:622-628 with src/cell_lists/full_grid.jl:84-94) of
+        one point given on the host, evaluated in the same types as the device."""
+        cl = self.cell_list
+        ct = self._cell_dtype.type if self._handle is not None else promote(
+            np.asarray(coords).dtype, cl.corner_dtype).type
+        cs = ct(self.search_radius)
+        out = []
+        for d in range(self.ndims):
+            with np.errstate(all="ignore"):
+                q = (ct(coords[d]) - ct(cl.min_corner[d])) / cs
+            out.append(wrap_int64(floor_to_int(float(q)) + 1))
+        return tuple(out)
+
+    # -- build -----------------------------------------------------------------------
+    def _build(self, y, eachindex_y):
+        nd = self.ndims
+        y = _device.as_coordinates(y, nd, "neighbor coordinates")
+        self._built = False
+        if float(self.search_radius) < FLOAT64_EPS:
+            # src/nhs_grid.jl:263-267: cannot initialize with zero search radius
+            self._empty = True
+            return
+        if self.cell_list.is_template:
+            raise PointNeighborsError("`search_radius` is not defined for this cell list")
+        self._empty = False
+        dt = _device.np_dtype(y)
+        self._ensure_handle(dt)
+        n = y.shape[1]
+        first, count, ptr, keep = _device.selection(eachindex_y, n, y.device)
+        self._set_stream(y)
+        lib = load()
+        st = lib.pn_grid_update(self._handle, ctypes.c_void_p(y.data_ptr()), n, first, count,
+                                ctypes.c_void_p(ptr) if ptr else None)
+        if st in (PN_ERR_OUT_OF_BOUNDS,):
+            p = ctypes.c_int64()
+            cell = (ctypes.c_int64 * nd)()
+            lib.pn_grid_error(self._handle, ctypes.byref(p), cell)
+            raise PointNeighborsError(self.cell_list.bounds_error_message(list(cell)))
+        check(st, "initialize!/update!")
+        del keep
+        self._built = True
+        self._y_key = (y.data_ptr(), tuple(y.shape), y.stride(), y._version)
+        self._y_full = _device.is_full(eachindex_y, n)
+        self._refreshed = False
+        self._n_y = n
+
+    def initialize(self, x, y, *, eachindex_y=None):
+        """initialize!(nhs, x, y; eachindex_y) (src/nhs_grid.jl:220-225)."""
+        self._build(y, eachindex_y)
+        return self
+
+    def update(self, x, y, *, points_moving=(True, True), eachindex_y=None):
+        """update!(nhs, x, y; points_moving, eachindex_y) (src/nhs_grid.jl:283-292): only
+        y's motion matters; the update is a full re-binning (:470-477)."""
+        if not points_moving[1]:
+            return self
+        self._build(y, eachindex_y)
+        return self
+
+    def copy(self, search_radius, n_points):
+        return copy_grid_search(self, search_radius, n_points)
+
+    def initialize_grid(self, y, *, parallelization_backend=None, eachindex_y=None):
+        self._build(y, eachindex_y)
+        return self
+
+    def update_grid(self, y, *, parallelization_backend=None, eachindex_y=None):
+        self._build(y, eachindex_y)
+        return self
+
+    # -- traversal support -------------------------------------------------------------
+    def _prepare(self, x, y, points):
+        """Returns the query mode for a traversal of x against the binned y."""
+        nd = self.ndims
+        x = _device.as_coordinates(x, nd, "system coordinates")
+        y = _device.as_coordinates(y, nd, "neighbor coordinates")
+        if self._empty or not self._built:
+            if self._empty:
+                raise BoundsError("the neighborhood search was initialized with a zero search radius")
+            raise PointNeighborsError("the neighborhood search is not initialized")
+        if _device.np_dtype(x) != self._coord_dtype or _device.np_dtype(y) != self._coord_dtype:
+            raise ArgumentError("system and neighbor coordinates must have the element type "
+                                "the neighborhood search was updated with")
+        lib = load()
+        self._set_stream(x)
+        y_key = (y.data_ptr(), tuple(y.shape), y.stride(), y._version)
+        if y_key != self._y_key or self._refreshed:
+            # cell membership from the last update!, coordinates read now
+            # (src/nhs_grid.jl:543-545)
+            if y.shape[1] != self._n_y:
+                raise BoundsError("neighbor coordinates have a different number of points than "
+                                  "the last update!")
+            check(lib.pn_grid_refresh(self._handle, ctypes.c_void_p(y.data_ptr()), y.shape[1]),
+                  "foreach_point_neighbor")
+            self._refreshed = y_key != self._y_key
+        nx = x.shape[1]
+        x_is_y = (_device.same_storage(x, y) and y_key == self._y_key and not self._refreshed)
+        if x_is_y and self._y_full and _device.is_full(points, nx):
+            return PN_QUERIES_GRID, x
+        first, count, ptr, keep = _device.selection(points, nx, x.device)
+        st = lib.pn_grid_bin_queries(self._handle, ctypes.c_void_p(x.data_ptr()), nx, first, count,
+                                     ctypes.c_void_p(ptr) if ptr else None)
+        check(st, "foreach_point_neighbor")
+        del keep
+        return PN_QUERIES_BINNED, x
+
+    def cell_table(self):
+        """(cell_start, sorted_index) of the last build as host numpy arrays: the cell
+        lists of the reference's DVoV (src/vector_of_vectors.jl:37-44) in CSR form."""
+        import torch
+        if not self._built:
+            raise PointNeighborsError("the neighborhood search is not initialized")
+        lib = load()
+        nb, ne, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib.pn_grid_stats(self._handle, ctypes.byref(nb), ctypes.byref(ne), ctypes.byref(mx)))
+        dev = torch.device("cuda", torch.cuda.current_device())
+        C = self.cell_list.n_cells_total
+        start = torch.empty(C + 1, dtype=torch.int32, device=dev)
+        idx = torch.empty(max(nb.value, 1), dtype=torch.int32, device=dev)
+        check(lib.pn_grid_copy_cells(self._handle, ctypes.c_void_p(start.data_ptr()),
+                                     ctypes.c_void_p(idx.data_ptr())))
+        torch.cuda.synchronize()
+        return start.cpu().numpy(), idx[: nb.value].cpu().numpy()
+
+    def points_in_cell(self, cell):
+        """Points of one cell (1-based cell tuple), src/nhs_grid.jl:593-597."""
+        start, idx = self.cell_table()
+        c = self.cell_list.cell_index(cell) - 1
+        return idx[start[c]:start[c + 1]].tolist()
+
+    def eachneighbor(self, coords):
+        """Candidate points of the 3^D cells around `coords` (src/nhs_grid.jl:585-591)."""
+        start, idx = self.cell_table()
+        cell = self.host_cell_coords(coords)
+        out = []
+        rng = [range(c - 1, c + 2) for c in cell]
+        for nc in itertools.product(*reversed(rng)):
+            nc = tuple(reversed(nc))
+            c = self.cell_list.cell_index(nc) - 1
+            out.extend(idx[start[c]:start[c + 1]].tolist())
+        return out
+
+    def stats(self):
+        lib = load()
+        nb, ne, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib.pn_grid_stats(self._handle, ctypes.byref(nb), ctypes.byref(ne), ctypes.byref(mx)))
+        return nb.value, ne.value, mx.value
+
+
+def initialize_grid(nhs, y, **kw):
+    return nhs.initialize_grid(y, **kw)
+
+
+def update_grid(nhs, y, **kw):
+    return nhs.update_grid(y, **kw)
+
+
+def copy_grid_search(nhs: GridNeighborhoodSearch, search_radius, n_points, eachpoint=None):
+    """copy_neighborhood_search (src/nhs_grid.jl:640-649)."""
+    cell_list = nhs.cell_list.copy_cell_list(search_radius, nhs.periodic_box)
+    return GridNeighborhoodSearch(nhs.ndims, search_radius=search_radius, n_points=n_points,
+                                  periodic_box=nhs.periodic_box, cell_list=cell_list,
+                                  update_strategy=nhs.update_strategy)

@@ -1,15 +1,18 @@
 """Benchmark of the neighbourhood-search hot path on one MI355X.
 
-Workload (one step): the 3D weakly-compressible-SPH neighbour step on 100^3 = 1,000,000
-jittered lattice points (spacing 0.01, jitter U[+-0.001], Float32), search radius
-2 * (1.2 * 0.01) (~58 neighbours per point): displace every point by a seeded
-U[+-0.0005], update! (counting-sort cell-list rebuild), then foreach_point_neighbor
-counting the neighbours of every point.  Inputs are resident in HBM when timing starts.
+Workload: 100^3 = 1,000,000 jittered lattice points (spacing 0.01, jitter U[+-0.001],
+Float32), smoothing length h = 1.2 * 0.01, search radius 2h (~58 neighbours per point).
+One step (default `--workload wcsph`, the weakly-compressible SPH step):
+    displace every point by a seeded U[+-0.0005]; update! (counting-sort rebuild);
+    cubic-spline density summation; Tait equation of state; pressure acceleration.
+`--workload count` replaces the physics by one neighbour-counting traversal.
+Inputs are resident in HBM when timing starts.
 
 Prints one JSON line.  `value` is neighbour pairs (ordered, self included) per second of
-whole steps.  `roofline` is the traversal kernel's algorithmic bytes per launch over its
-average launch time (HIP events on the launch stream), against 8 TB/s.  `cpu_baseline`
-times the oracle's threaded restatement of the reference CPU path on a bounded sample.
+whole steps.  `roofline` is the dominant traversal kernel's algorithmic bytes per launch
+over its average launch time (HIP events on the launch stream) against 8 TB/s.
+`cpu_baseline` times the oracle's threaded restatement of the reference CPU path on the
+same workload, on a bounded sample.
 """
 
 from __future__ import annotations
@@ -30,6 +33,7 @@ SHAPE = (100, 100, 100)
 SPACING = 0.01
 JITTER = 0.001
 DISP = 0.0005
+RHO0, SOUND, GAMMA = 1000.0, 10.0, 7.0
 
 
 def traversal_bytes(n_points: int, n_nonempty: int, d: int = 3, s: int = 4, f_in: int = 0,
@@ -45,6 +49,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["wcsph", "count", "batch"], default="wcsph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -56,44 +61,67 @@ def main():
 
     import torch
     import pointneighbors as pn
-    from pointneighbors import synthetic
+    from pointneighbors import sph, synthetic
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args, dev)))
+        return
     pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
     n = pts.shape[0]
-    r = np.float32(2.0) * (np.float32(1.2) * np.float32(SPACING))
+    h = np.float32(1.2) * np.float32(SPACING)
+    r = np.float32(2.0) * h
     lo = pts.min(axis=0) - np.float32(3) * r
     hi = pts.max(axis=0) + np.float32(3) * r
     cl = pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=r)
     nhs = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
     x = synthetic.to_device_coords(pts, dev)
-    counts = torch.zeros(n, dtype=torch.int32, device=dev)
     gen = torch.Generator(device=dev)
     gen.manual_seed(1234)
     disp = torch.empty((n, 3), dtype=torch.float32, device=dev)
-    f = pn.CountNeighbors(counts, accumulate=False)
     stream = torch.cuda.current_stream(dev)
+    mass = torch.full((n,), RHO0 * SPACING**3, dtype=torch.float32, device=dev)
+    rho = torch.zeros(n, dtype=torch.float32, device=dev)
+    P = torch.zeros(n, dtype=torch.float32, device=dev)
+    dv = torch.zeros((n, 3), dtype=torch.float32, device=dev).t()
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    f_count = pn.CountNeighbors(counts, accumulate=False)
+    f_rho = sph.SPHDensity(rho, mass, float(h), "cubic_spline", accumulate=False)
+    f_dv = sph.SPHPressureAcceleration(dv, mass, P, float(h), "cubic_spline", accumulate=False)
 
-    def step(time_kernels=None):
+    def step(ev=None):
         torch.rand((n, 3), generator=gen, device=dev, out=disp)
         disp.mul_(2 * DISP).sub_(DISP)
         x.add_(disp.t())
         pn.update(nhs, x, x)
-        if time_kernels is not None:
-            time_kernels[0].record(stream)
-        pn.foreach_point_neighbor(f, x, x, nhs)
-        if time_kernels is not None:
-            time_kernels[1].record(stream)
+        if args.workload == "count":
+            if ev is not None:
+                ev[0].record(stream)
+            pn.foreach_point_neighbor(f_count, x, x, nhs)
+            if ev is not None:
+                ev[1].record(stream)
+            return
+        if ev is not None:
+            ev[0].record(stream)
+        pn.foreach_point_neighbor(f_rho, x, x, nhs)
+        if ev is not None:
+            ev[1].record(stream)
+        p = sph.tait_eos(rho, RHO0, SOUND, GAMMA)
+        torch.div(p, rho * rho, out=P)
+        if ev is not None:
+            ev[2].record(stream)
+        pn.foreach_point_neighbor(f_dv, x, x, nhs)
+        if ev is not None:
+            ev[3].record(stream)
 
     pn.initialize(nhs, x, x)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    # timed region: exactly K whole steps
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
+    nev = 2 if args.workload == "count" else 4
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nev)] for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -104,27 +132,44 @@ def main():
     t_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    ms_total = t_start.elapsed_time(t_end)
-    ms_step = ms_total / args.steps
-    trav_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_step = t_start.elapsed_time(t_end) / args.steps
+
+    # pairs of the final configuration (outside the timed region)
+    pn.foreach_point_neighbor(f_count, x, x, nhs)
+    torch.cuda.synchronize()
     pairs = int(counts.sum().item())
     n_binned, n_nonempty, max_count = nhs.stats()
 
-    algo = traversal_bytes(n, n_nonempty)
+    extra = {}
+    if args.workload == "count":
+        trav_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        kernel = "k_traverse<3,float,float,CountF>"
+        algo = traversal_bytes(n, n_nonempty)
+        extra["count_traversal_ms"] = trav_ms
+    else:
+        dens_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        eos_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+        trav_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+        kernel = "k_traverse<3,float,float,PressureAccelF>"
+        # per neighbour: coordinates + m_j + P_j; per query: P_i; out: 3 floats
+        algo = traversal_bytes(n, n_nonempty, f_in=8, f_own=4, f_out=12)
+        extra.update({"density_traversal_ms": dens_ms, "eos_ms": eos_ms, "pressure_traversal_ms": trav_ms,
+                      "density_pairs_per_s": pairs / (dens_ms * 1e-3),
+                      "pressure_pairs_per_s": pairs / (trav_ms * 1e-3)})
     achieved = algo / (trav_ms * 1e-3) / 1e9
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traverse_count_1m_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_1m.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(pts, lo, hi, r, args.cpu_seconds)
+        cpu = cpu_baseline(pts, lo, hi, r, h, args.workload, args.cpu_seconds)
 
     line = {
-        "metric": "neighbour pairs/s (1M 3D Float32 points, update! + count per step)",
+        "metric": "neighbour pairs/s at 1M 3D Float32 points (whole step) + traversal % of HBM peak",
         "value": pairs / (ms_step * 1e-3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -136,28 +181,137 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 jittered lattice, torch Philox displacements)",
-        "config": {"workload": "wcsph-1m-3d-f32: 100^3 lattice, spacing 0.01, jitter 0.001, "
-                               "r = 2*(1.2*0.01), step = displace U[+-0.0005] + update! + "
-                               "count neighbours",
+        "config": {"workload": ("wcsph-1m-3d-f32: 100^3 lattice, spacing 0.01, jitter 0.001, h = 1.2*0.01, "
+                                "r = 2h; step = displace U[+-0.0005] + update! + cubic-spline density + "
+                                "Tait EOS + pressure acceleration") if args.workload == "wcsph" else
+                               ("count-1m-3d-f32: 100^3 lattice, spacing 0.01, jitter 0.001, r = 2*(1.2*0.01); "
+                                "step = displace U[+-0.0005] + update! + count neighbours"),
                    "n_points": n, "search_radius": float(r), "n_cells": int(np.prod(cl.n_cells)),
                    "nonempty_cells": int(n_nonempty), "max_cell_population": int(max_count)},
         "pairs_per_step": pairs,
-        "traversal_ms": trav_ms,
-        "traversal_pairs_per_s": pairs / (trav_ms * 1e-3),
-        "update_and_displace_ms": ms_step - trav_ms,
+        "dominant_traversal_ms": trav_ms,
+        **extra,
         "wall_s": wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_traverse<3,float,float,CountF>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "algorithmic_bytes_per_launch": algo},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
 
-def cpu_baseline(pts, lo, hi, r, budget_s):
+BATCH_CLOUDS, BATCH_MIN, BATCH_MAX, BATCH_R = 512, 1000, 65536, 2.4
+
+
+def batch_clouds(seed=1):
+    """512 jittered lattices (spacing 1, jitter U[+-0.25], Float32), sizes log-uniform in
+    [1000, 65536], each in its own box."""
+    from pointneighbors import synthetic
+    u = synthetic.uniform(seed, BATCH_CLOUDS)
+    sizes = np.round(np.exp(np.log(BATCH_MIN) + u * (np.log(BATCH_MAX) - np.log(BATCH_MIN)))).astype(int)
+    clouds = []
+    for c, s_ in enumerate(sizes):
+        side = max(2, int(round(s_ ** (1 / 3))))
+        shape = (side, side, max(1, int(round(s_ / side / side))))
+        clouds.append(synthetic.jittered_lattice(shape, 1.0, 0.25, dtype=np.float32, seed=seed * 100003 + c))
+    return clouds
+
+
+def bench_batch(args, dev):
+    """512 uneven clouds, 100-step loop of displace U[+-0.05] + update! + count: per-cloud
+    launches (one GridNeighborhoodSearch per cloud, no synchronization inside the step)
+    against one batched search whose step is captured in a graph."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import batch, synthetic
+    clouds = batch_clouds()
+    r = np.float32(BATCH_R)
+    margin = np.float32(3 * BATCH_R)
+    cls = [pn.FullGridCellList(min_corner=p.min(axis=0) - margin, max_corner=p.max(axis=0) + margin,
+                               search_radius=r) for p in clouds]
+    offs = np.concatenate([[0], np.cumsum([len(p) for p in clouds])])
+    n = int(offs[-1])
+    allpts = np.concatenate(clouds)
+    steps = max(args.steps, 1)
+    stream = torch.cuda.current_stream(dev)
+
+    # per-cloud launches
+    y1 = synthetic.to_device_coords(allpts, dev)
+    c1 = torch.zeros(n, dtype=torch.int32, device=dev)
+    step1 = torch.zeros(1, dtype=torch.int64, device=dev)
+    nhss, views, outs = [], [], []
+    for c in range(len(clouds)):
+        v = y1[:, offs[c]:offs[c + 1]]
+        nh = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=len(clouds[c]), cell_list=cls[c])
+        pn.initialize(nh, v, v)
+        nhss.append(nh)
+        views.append(v)
+        outs.append(pn.CountNeighbors(c1[offs[c]:offs[c + 1]], accumulate=False))
+
+    def per_cloud_step():
+        batch.displace_uniform(y1, 0.05, 7, step1)
+        for c in range(len(clouds)):
+            pn.update(nhss[c], views[c], views[c], check_errors=False)
+            pn.foreach_point_neighbor(outs[c], views[c], views[c], nhss[c])
+
+    # one batched search, step captured in a graph
+    nb = batch.BatchedGridNeighborhoodSearch(3, search_radius=r, cell_lists=cls, n_points=[len(p) for p in clouds])
+    y2 = synthetic.to_device_coords(allpts, dev)
+    c2 = torch.zeros(n, dtype=torch.int32, device=dev)
+    step2 = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def batched_step():
+        batch.displace_uniform(y2, 0.05, 7, step2)
+        nb.update(y2, check_errors=False)
+        nb.count_neighbors(c2)
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(k):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / k
+
+    per_cloud_ms = timed(per_cloud_step, min(steps, 20))
+    for nh in nhss:
+        nh.check()
+    eager_ms = timed(batched_step, steps)
+    nb.check()
+    s = torch.cuda.Stream(dev)
+    s.wait_stream(stream)
+    with torch.cuda.stream(s):
+        batched_step()
+    stream.wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batched_step()
+    graph_ms = timed(g.replay, steps)
+    nb.check()
+    pairs = int(c2.sum().item())
+    return {
+        "metric": "batched update! + count of 512 uneven clouds: speedup over per-cloud launches",
+        "value": per_cloud_ms / graph_ms, "unit": "x", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded SplitMix64 jittered lattices and displacements)",
+        "config": {"workload": "batch-512-clouds: sizes log-uniform 1000..65536, spacing 1, jitter 0.25, "
+                               "r = 2.4; step = displace U[+-0.05] + update! + count",
+                   "n_clouds": len(clouds), "n_points": n},
+        "per_cloud_ms_per_step": per_cloud_ms, "batched_eager_ms_per_step": eager_ms,
+        "batched_graph_ms_per_step": graph_ms, "pairs_per_step": pairs,
+        "pairs_per_s": pairs / (graph_ms * 1e-3),
+    }
+
+
+def cpu_baseline(pts, lo, hi, r, h, workload, budget_s):
     """The oracle's threaded restatement of the reference CPU path (ParallelUpdate build
-    with atomic pushes + point-major traversal statically chunked over points) on the
+    with atomic pushes + point-major traversals statically chunked over points) on the
     same 1M-point workload, repeated until about `budget_s` seconds of work."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as orc
@@ -167,24 +321,29 @@ def cpu_baseline(pts, lo, hi, r, budget_s):
     cores = max(1, min(cores, 16))
     g = orc.Grid(3, cl.min_corner.astype(np.float64), cl.n_cells, float(r), coord_dtype=np.float32,
                  radius_dtype=np.float32, cell_dtype=np.float32, max_points_per_cell=100)
-    reps, t_build, t_trav, pairs = 0, 0.0, 0.0, 0
+    reps, t_build, t_work, pairs = 0, 0.0, 0.0, 0
     t0 = time.perf_counter()
     while True:
-        st, counts, tb, tt = g.threaded_count_f32(pts, cores)
+        if workload == "count":
+            st, counts, tb, tt = g.threaded_count_f32(pts, cores)
+            pairs = int(counts.sum()) if st == 0 else 0
+        else:
+            st, rho, dv, tb, tt = g.threaded_wcsph_f32(pts, cores, RHO0 * SPACING**3, float(h), RHO0, SOUND, GAMMA)
+            if st == 0 and pairs == 0:
+                _, counts, _, _ = g.threaded_count_f32(pts, cores)
+                pairs = int(counts.sum())
         if st != 0:
             return None
         reps += 1
         t_build += tb
-        t_trav += tt
-        pairs = int(counts.sum())
+        t_work += tt
         if time.perf_counter() - t0 > budget_s or reps >= 50:
             break
-    per_step = (t_build + t_trav) / reps
+    per_step = (t_build + t_work) / reps
     return {"value": pairs / per_step, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} x (update! + count) of the same 1M-point Float32 workload "
-                      f"(one fixed configuration, no displacement); build {t_build / reps * 1e3:.1f} ms, "
-                      f"traversal {t_trav / reps * 1e3:.1f} ms per step",
-            "traversal_pairs_per_s": pairs / (t_trav / reps)}
+            "sample": f"{reps} steps (update! + {'count' if workload == 'count' else 'density + EOS + pressure'}) "
+                      f"of the same 1M-point Float32 configuration, no displacement; build "
+                      f"{t_build / reps * 1e3:.1f} ms, traversals {t_work / reps * 1e3:.1f} ms per step"}
 
 
 if __name__ == "__main__":

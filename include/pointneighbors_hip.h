@@ -167,6 +167,65 @@ int pn_list_sum_neighbor_indices(const int32_t* row_ptr, const int32_t* col_idx,
                                  const int32_t* sel_list, int64_t* sums, int index_base,
                                  int accumulate, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * Interaction functors and their adjoints.  No reference counterpart in PointNeighbors.jl:
+ * these are the closures a WCSPH step passes to foreach_point_neighbor (the workload of
+ * benchmarks/smoothed_particle_hydrodynamics.jl:45-102, physics from TrixiParticles) and a
+ * Wendland C2 pair energy, plus hand-written reverse-mode kernels that recompute the
+ * traversal instead of storing pair lists.  Fields are arrays of the radius type indexed
+ * by original point index; vector fields (dv, x_bar, dv_bar) are D x N like coordinates.
+ * kernel: PN_KERNEL_CUBIC_SPLINE (Monaghan) or PN_KERNEL_WENDLAND_C2, support 2h.
+ * Functions without a `queries` argument need x == y (the grid's own points).  Enqueue only.
+ *   pn_sph_density            rho_i = sum_j m_j W(r_ij)  (self included)
+ *   pn_sph_pressure_accel     dv_i = -sum_{j != i} m_j (P_i + P_j) grad W_ij,  P = p / rho^2
+ *   pn_pair_energy            e_i = 1/2 sum_{j != i} eps (1 - r/rc)^4 (1 + 4 r/rc), r < rc
+ *   *_vjp                     vector-Jacobian products for upstream gradients of the outputs
+ *   pn_tait_eos               p = B ((rho/rho0)^gamma - 1), B = rho0 c^2 / gamma
+ * ---------------------------------------------------------------------------------- */
+enum pn_kernel { PN_KERNEL_CUBIC_SPLINE = 0, PN_KERNEL_WENDLAND_C2 = 1 };
+
+int pn_sph_density(pn_grid_t grid, int queries, int kernel, double h, const void* mass,
+                   void* rho, int accumulate);
+int pn_sph_density_vjp(pn_grid_t grid, int kernel, double h, const void* mass,
+                       const void* rho_bar, void* x_bar, void* mass_bar, int accumulate);
+int pn_sph_pressure_accel(pn_grid_t grid, int kernel, double h, const void* mass, const void* P,
+                          void* dv, int accumulate);
+int pn_sph_pressure_accel_vjp(pn_grid_t grid, int kernel, double h, const void* mass,
+                              const void* P, const void* dv_bar, void* x_bar, void* P_bar,
+                              void* mass_bar, int accumulate);
+int pn_pair_energy(pn_grid_t grid, int queries, double eps, double cutoff, void* e,
+                   int accumulate);
+int pn_pair_energy_vjp(pn_grid_t grid, double eps, double cutoff, const void* e_bar,
+                       void* x_bar, int accumulate);
+int pn_tait_eos(const void* rho, void* p, int64_t n, int dtype, double rho0, double c,
+                double gamma, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Batches of independent clouds.  No reference counterpart: the nearest concept is one
+ * GridNeighborhoodSearch per point set made with copy_neighborhood_search
+ * (src/neighborhood_search.jl:107-110, src/nhs_grid.jl:640-649), launched per cloud.
+ * pn_batch_create returns a grid whose cell table concatenates the clouds' grids (same
+ * radius, own padded min corner and n_cells each, computed as FullGridCellList does);
+ * points are the concatenation of the clouds (cloud c owns [point_offsets[c],
+ * point_offsets[c+1])).  One pn_grid_update(_async) bins every cloud, one traversal call
+ * serves every cloud (PN_QUERIES_GRID only).  Launch shapes do not depend on the data, so
+ * an update+query step can be captured in a hipGraph (use pn_grid_update_async inside).
+ *   min_corners, n_cells: n_clouds x ndims, row-major per cloud
+ * ---------------------------------------------------------------------------------- */
+int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype, int64_t n_clouds,
+                    const double* min_corners, const int64_t* n_cells, double search_radius,
+                    int32_t max_points_per_cell, const int64_t* point_offsets, pn_grid_t* out);
+
+/* Tuning knobs (the autotuner's choices).  No reference counterpart.
+ *   "persist_blocks"  workgroups of the persistent traversal grid (multiple of 8) */
+int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value);
+
+/* Adds U[-amplitude, amplitude] to each of the n values of x (a pure function of seed,
+ * *step and the index), then increments *step on the device: a replayable per-step
+ * perturbation like benchmarks/update.jl:28-49.  Enqueue only. */
+int pn_displace_uniform(void* x, int64_t n, int dtype, double amplitude, uint64_t seed,
+                        int64_t* step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

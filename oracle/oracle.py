@@ -59,6 +59,18 @@ def lib():
         L.orc_precomputed_lists.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp]
         L.orc_threaded_count_f32.restype = ci
         L.orc_threaded_count_f32.argtypes = [vp, vp, i64, ci, vp, vp, vp]
+        L.orc_threaded_wcsph_f32.restype = ci
+        L.orc_threaded_wcsph_f32.argtypes = [vp, vp, i64, ci, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_float, vp, vp, vp, vp]
+        L.orc_sph_density.restype = None
+        L.orc_sph_density.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_sph_pressure_accel.restype = None
+        L.orc_sph_pressure_accel.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp, vp]
+        L.orc_pair_energy.restype = None
+        L.orc_pair_energy.argtypes = [ci, dbl, dbl, vp, i64, vp, vp, vp]
+        L.orc_loss_dual.restype = None
+        L.orc_loss_dual.argtypes = [ci, ci, ci, dbl, dbl, dbl, vp, i64, vp, vp, vp, dbl, dbl, dbl, i64, ci,
+                                    vp, vp]
         _lib = L
     return _lib
 
@@ -172,6 +184,15 @@ class Grid:
                                          _ptr(rowptr), _ptr(cols), ctypes.byref(bpc))
         return st, bpc.value, (rowptr, cols[: rowptr[-1]])
 
+    def threaded_wcsph_f32(self, y_nd, nthreads, mass, h, rho0, c, gamma=7.0):
+        y = self._pts(y_nd)
+        rho = np.zeros(y.shape[0], dtype=np.float32)
+        dv = np.zeros_like(y)
+        tb, tp = ctypes.c_double(), ctypes.c_double()
+        st = lib().orc_threaded_wcsph_f32(self.h, _ptr(y), y.shape[0], int(nthreads), mass, h, rho0, c, gamma,
+                                          _ptr(rho), _ptr(dv), ctypes.byref(tb), ctypes.byref(tp))
+        return st, rho, dv, tb.value, tp.value
+
     def threaded_count_f32(self, y_nd, nthreads):
         y = self._pts(y_nd)
         counts = np.zeros(y.shape[0], dtype=np.int32)
@@ -221,3 +242,68 @@ def sorted_lists(rowptr, cols):
     rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
     order = np.lexsort((cols, rows))
     return cols[order]
+
+
+# ---- physics over fixed neighbour lists (x == y), double precision ---------------------------
+
+KERNELS = {"cubic_spline": 0, "wendland_c2": 1}
+
+
+def _csr(lists):
+    rowptr, cols = lists
+    return np.ascontiguousarray(rowptr, dtype=np.int64), np.ascontiguousarray(cols, dtype=np.int32)
+
+
+def sph_density(x_nd, lists, mass, h, kernel="cubic_spline"):
+    """(rho, scale) with scale_i = sum_j |m_j W_ij|."""
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    rho = np.zeros(x.shape[0])
+    scale = np.zeros(x.shape[0])
+    lib().orc_sph_density(x.shape[1], KERNELS[kernel], float(h), _ptr(x), x.shape[0], _ptr(rp), _ptr(cols),
+                          _ptr(m), _ptr(rho), _ptr(scale))
+    return rho, scale
+
+
+def sph_pressure_accel(x_nd, lists, mass, P, h, kernel="cubic_spline"):
+    """(dv (N, D), scale) with scale_i = sum_j sum_d |term|."""
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    Pa = np.ascontiguousarray(P, dtype=np.float64)
+    dv = np.zeros_like(x)
+    scale = np.zeros(x.shape[0])
+    lib().orc_sph_pressure_accel(x.shape[1], KERNELS[kernel], float(h), _ptr(x), x.shape[0], _ptr(rp), _ptr(cols),
+                                 _ptr(m), _ptr(Pa), _ptr(dv), _ptr(scale))
+    return dv, scale
+
+
+def pair_energy(x_nd, lists, eps, rc):
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    e = np.zeros(x.shape[0])
+    lib().orc_pair_energy(x.shape[1], float(eps), float(rc), _ptr(x), x.shape[0], _ptr(rp), _ptr(cols), _ptr(e))
+    return e
+
+
+LOSS_ENERGY, LOSS_DENSITY, LOSS_FORCE = 0, 1, 2
+
+
+def loss_dual(which, x_nd, lists, k, dim, *, mass=None, kernel="cubic_spline", h=1.0, eps=1.0, rc=1.0,
+              rho0=1.0, c=1.0, gamma=7.0):
+    """(loss, d loss / d x[k, dim]) by forward-mode dual numbers."""
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(np.ones(x.shape[0]) if mass is None else mass, dtype=np.float64)
+    v, d = ctypes.c_double(), ctypes.c_double()
+    lib().orc_loss_dual(which, x.shape[1], KERNELS[kernel], float(h), float(eps), float(rc), _ptr(x), x.shape[0],
+                        _ptr(rp), _ptr(cols), _ptr(m), float(rho0), float(c), float(gamma), int(k), int(dim),
+                        ctypes.byref(v), ctypes.byref(d))
+    return v.value, d.value
+
+
+def tait_eos(rho, rho0, c, gamma=7.0):
+    B = rho0 * c * c / gamma
+    return B * ((np.asarray(rho, dtype=np.float64) / rho0) ** gamma - 1.0)
+

@@ -524,3 +524,307 @@ int orc_threaded_count_f32(void* h, const float* y, int64_t n, int nthreads, int
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Interaction physics over fixed neighbour lists (CSR from orc_traverse, x == y).
+// Not in the reference (TrixiParticles physics): textbook formulas, evaluated in double
+// or in forward-mode dual numbers (value + one directional derivative), which stand in
+// for ForwardDiff through the CPU path.
+//   cubic spline (Monaghan 1992) and Wendland C2 (Wendland 1995) kernels, support 2h
+//   Tait EOS p = B((rho/rho0)^gamma - 1), B = rho0 c^2/gamma
+//   a_i = -sum_{j != i} m_j (P_i + P_j) W'(r) (x_i - x_j)/r,  P = p/rho^2
+//   phi(r) = eps (1 - r/rc)^4 (1 + 4 r/rc)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double a) : v(a), d(0.0) {}
+    Dual(double a, double b) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline Dual sqrt(Dual a) {
+    double s = std::sqrt(a.v);
+    return {s, s > 0 ? a.d / (2.0 * s) : 0.0};
+}
+inline Dual pow(Dual a, double e) { return {std::pow(a.v, e), e * std::pow(a.v, e - 1.0) * a.d}; }
+inline double val(double a) { return a; }
+inline double val(Dual a) { return a.v; }
+inline double absval(double a) { return std::fabs(a); }
+using std::pow;
+using std::sqrt;
+
+template <class T>
+struct Kern {
+    int kind;  // 0 cubic spline, 1 Wendland C2
+    int nd;
+    double h;
+    double sigma() const {
+        const double pi = 3.14159265358979323846;
+        if (kind == 0) return nd == 3 ? 1.0 / (pi * h * h * h) : 10.0 / (7.0 * pi * h * h);
+        return nd == 3 ? 21.0 / (16.0 * pi * h * h * h) : 7.0 / (4.0 * pi * h * h);
+    }
+    T w(T r) const {
+        T q = r / T(h);
+        double s = sigma();
+        if (kind == 0) {
+            if (q < T(1.0)) return T(s) * (T(1.0) - T(1.5) * q * q + T(0.75) * q * q * q);
+            if (q < T(2.0)) {
+                T t = T(2.0) - q;
+                return T(s * 0.25) * t * t * t;
+            }
+            return T(0.0);
+        }
+        if (q < T(2.0)) {
+            T t = T(1.0) - T(0.5) * q;
+            return T(s) * t * t * t * t * (T(2.0) * q + T(1.0));
+        }
+        return T(0.0);
+    }
+    T dw(T r) const {
+        T q = r / T(h);
+        double s = sigma() / h;
+        if (kind == 0) {
+            if (q < T(1.0)) return T(s) * (T(-3.0) * q + T(2.25) * q * q);
+            if (q < T(2.0)) {
+                T t = T(2.0) - q;
+                return T(-0.75 * s) * t * t;
+            }
+            return T(0.0);
+        }
+        if (q < T(2.0)) {
+            T t = T(1.0) - T(0.5) * q;
+            return T(-5.0 * s) * q * t * t * t;
+        }
+        return T(0.0);
+    }
+};
+
+template <class T>
+struct Phys {
+    int nd;
+    const T* x;  // N x nd
+    int64_t n;
+    const int64_t* rp;
+    const int32_t* cols;
+
+    T dist(int64_t i, int64_t j, T pd[3]) const {
+        T d2 = T(0.0);
+        for (int d = 0; d < nd; ++d) {
+            pd[d] = x[nd * i + d] - x[nd * j + d];
+            d2 += pd[d] * pd[d];
+        }
+        return d2;
+    }
+
+    void density(const Kern<T>& k, const T* mass, T* rho, double* scale) const {
+        for (int64_t i = 0; i < n; ++i) {
+            T s = T(0.0);
+            double a = 0.0;
+            for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+                int32_t j = cols[e];
+                T pd[3];
+                T r = sqrt(dist(i, j, pd));
+                T t = mass[j] * k.w(r);
+                s += t;
+                a += std::fabs(val(t));
+            }
+            rho[i] = s;
+            if (scale) scale[i] = a;
+        }
+    }
+
+    void pressure(const Kern<T>& k, const T* mass, const T* P, T* dv, double* scale) const {
+        for (int64_t i = 0; i < n; ++i) {
+            T acc[3] = {T(0.0), T(0.0), T(0.0)};
+            double a = 0.0;
+            for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+                int32_t j = cols[e];
+                T pd[3];
+                T d2 = dist(i, j, pd);
+                if (!(val(d2) > 0.0)) continue;
+                T r = sqrt(d2);
+                T c = -mass[j] * (P[i] + P[j]) * k.dw(r) / r;
+                for (int d = 0; d < nd; ++d) {
+                    acc[d] += c * pd[d];
+                    a += std::fabs(val(c * pd[d]));
+                }
+            }
+            for (int d = 0; d < nd; ++d) dv[nd * i + d] = acc[d];
+            if (scale) scale[i] = a;
+        }
+    }
+
+    void energy(double eps, double rc, T* e) const {
+        for (int64_t i = 0; i < n; ++i) {
+            T s = T(0.0);
+            for (int64_t ed = rp[i]; ed < rp[i + 1]; ++ed) {
+                int32_t j = cols[ed];
+                T pd[3];
+                T d2 = dist(i, j, pd);
+                if (!(val(d2) > 0.0)) continue;
+                T q = sqrt(d2) / T(rc);
+                if (q < T(1.0)) {
+                    T t = T(1.0) - q;
+                    s += T(eps) * t * t * t * t * (T(1.0) + T(4.0) * q);
+                }
+            }
+            e[i] = T(0.5) * s;
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Density, pressure acceleration and pair energy in double over CSR neighbour lists.
+// scale (nullable) receives sum_j |term_j| per point: the summation-error scale.
+void orc_sph_density(int nd, int kernel, double h, const double* x, int64_t n, const int64_t* rp,
+                     const int32_t* cols, const double* mass, double* rho, double* scale) {
+    Phys<double> p{nd, x, n, rp, cols};
+    p.density(Kern<double>{kernel, nd, h}, mass, rho, scale);
+}
+
+void orc_sph_pressure_accel(int nd, int kernel, double h, const double* x, int64_t n, const int64_t* rp,
+                            const int32_t* cols, const double* mass, const double* P, double* dv, double* scale) {
+    Phys<double> p{nd, x, n, rp, cols};
+    p.pressure(Kern<double>{kernel, nd, h}, mass, P, dv, scale);
+}
+
+void orc_pair_energy(int nd, double eps, double rc, const double* x, int64_t n, const int64_t* rp,
+                     const int32_t* cols, double* e) {
+    Phys<double> p{nd, x, n, rp, cols};
+    p.energy(eps, rc, e);
+}
+
+// Scalar losses in dual numbers, derivative along coordinate (k, dim) (forward mode):
+//   which 0: E = sum_i e_i (Wendland pair energy, eps, rc = cutoff)
+//   which 1: L = sum_i (rho_i - rho0)^2, rho with `kernel`, mass m
+//   which 2: L = sum_i |a_i|^2, a from rho -> Tait EOS (rho0, c, gamma) -> P = p/rho^2
+// Writes the loss and dL/dx[k][dim].
+void orc_loss_dual(int which, int nd, int kernel, double h, double eps, double rc, const double* x, int64_t n,
+                   const int64_t* rp, const int32_t* cols, const double* mass, double rho0, double c, double gamma,
+                   int64_t k, int dim, double* value, double* deriv) {
+    std::vector<Dual> xd((size_t)(n * nd));
+    for (int64_t i = 0; i < n * nd; ++i) xd[i] = Dual(x[i], 0.0);
+    xd[(size_t)(k * nd + dim)].d = 1.0;
+    std::vector<Dual> md((size_t)n);
+    for (int64_t i = 0; i < n; ++i) md[i] = Dual(mass[i]);
+    Phys<Dual> p{nd, xd.data(), n, rp, cols};
+    Dual L(0.0);
+    if (which == 0) {
+        std::vector<Dual> e((size_t)n);
+        p.energy(eps, rc, e.data());
+        for (auto& v : e) L += v;
+    } else {
+        std::vector<Dual> rho((size_t)n);
+        Kern<Dual> kern{kernel, nd, h};
+        p.density(kern, md.data(), rho.data(), nullptr);
+        if (which == 1) {
+            for (auto& r : rho) L += (r - Dual(rho0)) * (r - Dual(rho0));
+        } else {
+            const double B = rho0 * c * c / gamma;
+            std::vector<Dual> P((size_t)n), dv((size_t)(n * nd));
+            for (int64_t i = 0; i < n; ++i) {
+                Dual pr = Dual(B) * (pow(rho[i] / Dual(rho0), gamma) - Dual(1.0));
+                P[i] = pr / (rho[i] * rho[i]);
+            }
+            p.pressure(kern, md.data(), P.data(), dv.data(), nullptr);
+            for (auto& v : dv) L += v * v;
+        }
+    }
+    *value = L.v;
+    *deriv = L.d;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Threaded CPU WCSPH step with the reference CPU path's structure (Float32, 3D):
+// ParallelUpdate build with atomic pushes, then two point-major traversals statically
+// chunked over points (density, then pressure acceleration), the Tait EOS in between.
+// Timings (seconds) of the build and of the two traversals + EOS go to the out pointers.
+int orc_threaded_wcsph_f32(void* h, const float* y, int64_t n, int nthreads, float mass, float hsml, float rho0,
+                           float c, float gamma, float* rho, float* dv, double* t_build, double* t_phys) {
+    OracleGrid& og = *(OracleGrid*)h;
+    if (og.tc != 0 || og.tr != 0 || og.g.ndims != 3) return ERR_ARG;
+    std::vector<int32_t> counts((size_t)n);
+    double tb = 0, tt = 0;
+    int st = orc_threaded_count_f32(h, y, n, nthreads, counts.data(), &tb, &tt);
+    if (st != OK) return st;
+    const Geom& g = og.g;
+    const float r = (float)og.radius;
+    const double pi = 3.14159265358979323846;
+    const float sigma = (float)(1.0 / (pi * (double)hsml * hsml * hsml));
+    const float inv_h = 1.0f / hsml;
+    auto now = [] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    auto chunked = [&](int64_t count, auto&& body) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; ++t) {
+            int64_t b = count * t / nthreads, e = count * (t + 1) / nthreads;
+            th.emplace_back([=, &body] {
+                for (int64_t i = b; i < e; ++i) body(i);
+            });
+        }
+        for (auto& x : th) x.join();
+    };
+    auto w = [&](float d) {
+        float q = d * inv_h;
+        if (q < 1.0f) return sigma * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+        if (q < 2.0f) {
+            float t = 2.0f - q;
+            return sigma * 0.25f * t * t * t;
+        }
+        return 0.0f;
+    };
+    auto dw = [&](float d) {
+        float q = d * inv_h;
+        if (q < 1.0f) return sigma * inv_h * (-3.0f * q + 2.25f * q * q);
+        if (q < 2.0f) {
+            float t = 2.0f - q;
+            return sigma * inv_h * (-0.75f * t * t);
+        }
+        return 0.0f;
+    };
+    std::vector<float> P((size_t)n);
+    const float B = rho0 * c * c / gamma;
+    double t0 = now();
+    chunked(n, [&](int64_t i) {
+        float s = 0.0f;
+        traverse_point<float, float, float>(g, og.cs, y, y + 3 * i, r,
+                                            [&](int32_t, const float*, float d2) { s += mass * w(std::sqrt(d2)); });
+        rho[i] = s;
+        float p = B * (std::pow(s / rho0, gamma) - 1.0f);
+        P[i] = p / (s * s);
+    });
+    chunked(n, [&](int64_t i) {
+        float a[3] = {0, 0, 0};
+        traverse_point<float, float, float>(g, og.cs, y, y + 3 * i, r, [&](int32_t j, const float* pd, float d2) {
+            if (d2 > 0.0f) {
+                float d = std::sqrt(d2);
+                float cc = -mass * (P[i] + P[j]) * dw(d) / d;
+                for (int k = 0; k < 3; ++k) a[k] += cc * pd[k];
+            }
+        });
+        for (int k = 0; k < 3; ++k) dv[3 * i + k] = a[k];
+    });
+    double t1 = now();
+    if (t_build) *t_build = tb;
+    if (t_phys) *t_phys = t1 - t0;
+    return OK;
+}
+
+}  // extern "C"

@@ -8,9 +8,13 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/pointneighbors_hip.h"
-#include "pn_kernels.h"
+#include "pn_build.h"
+#include "pn_lists.h"
+#include "pn_sph.h"
+#include "pn_traverse.h"
 
 namespace {
 
@@ -132,6 +136,12 @@ struct pn_grid {
     hipStream_t stream = nullptr;
     BinSet grid;
     BinSet query;
+    Buf fields;  // per-neighbour fields gathered into the grid's sorted order
+    // batch of independent clouds: per-cloud geometry, point -> cloud, cell offsets
+    Buf geom_table, cloud_of, cell_offs;
+    std::vector<pn::GeomEntry> host_table;
+    std::vector<int64_t> host_point_offsets;
+    int64_t persist_blocks = 4096;  // workgroups of the persistent traversal (multiple of 8)
     Buf csr_counts;
     Buf csr_status;
     Scanner csr_scan;
@@ -141,6 +151,10 @@ struct pn_grid {
     const void* err_src = nullptr;
 
     ~pn_grid() {
+        fields.release();
+        geom_table.release();
+        cloud_of.release();
+        cell_offs.release();
         csr_counts.release();
         csr_status.release();
         csr_scan.block_sums.release();
@@ -275,12 +289,18 @@ int describe_point(pn_grid& g, const void* src, int64_t p) {
     } else {
         PN_HIP(hipMemcpy(xs, (const double*)src + g.nd * p, g.nd * 8, hipMemcpyDeviceToHost));
     }
+    const double* minc = g.geom.minc;
+    if (!g.host_table.empty()) {
+        auto it = std::upper_bound(g.host_point_offsets.begin(), g.host_point_offsets.end(), p);
+        const size_t cloud = (size_t)(it - g.host_point_offsets.begin()) - 1;
+        minc = g.host_table[std::min(cloud, g.host_table.size() - 1)].minc;
+    }
     for (int d = 0; d < g.nd; ++d) {
         double q;
         if (g.tcell == PN_FLOAT32)
-            q = (double)(((float)xs[d] - (float)g.geom.minc[d]) / (float)g.geom.cs);
+            q = (double)(((float)xs[d] - (float)minc[d]) / (float)g.geom.cs);
         else
-            q = (xs[d] - g.geom.minc[d]) / g.geom.cs;
+            q = (xs[d] - minc[d]) / g.geom.cs;
         double r = std::floor(q);
         int64_t c;
         if (std::isnan(r) || r >= 9223372036854775808.0) c = INT64_MAX;
@@ -383,6 +403,8 @@ int pn_grid_set_stream(pn_grid_t grid, void* stream) {
 int pn_grid_update_async(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first, int64_t sel_count,
                          const int32_t* sel_list) {
     if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (grid->geom.table && n_y != grid->host_point_offsets.back())
+        return fail(PN_ERR_ARGUMENT, "a batch grid is updated with the concatenated points of all its clouds");
     grid->err_src = y;
     return enqueue_bin(*grid, grid->grid, y, n_y, sel_first, sel_count, sel_list);
 }
@@ -431,6 +453,7 @@ int pn_grid_refresh(pn_grid_t grid, const void* y, int64_t n_y) {
 int pn_grid_bin_queries(pn_grid_t grid, const void* x, int64_t n_x, int64_t sel_first, int64_t sel_count,
                         const int32_t* sel_list) {
     if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (grid->geom.table) return fail(PN_ERR_UNSUPPORTED, "a batch grid traverses its own points only");
     int rc = enqueue_bin(*grid, grid->query, x, n_x, sel_first, sel_count, sel_list);
     if (rc) {
         grid->query.valid = false;
@@ -479,37 +502,80 @@ const BinSet* query_set(pn_grid& g, int queries) {
     return nullptr;
 }
 
-// Candidate chunk staged in LDS per wavefront.
-template <class TC>
+// Candidate chunk staged in LDS per wavefront.  Functors with per-neighbour fields and
+// pair queues use smaller chunks: LDS per wavefront bounds the waves per CU.
+template <class TC, class F>
 struct Chunk {
-    static constexpr int value = sizeof(TC) == 4 ? 512 : 256;
+    static constexpr int light = sizeof(TC) == 4 ? 512 : 256;
+    static constexpr int heavy = sizeof(TC) == 4 ? 256 : 128;
+    static constexpr int value = F::kQueue ? heavy : light;
 };
 
-template <class F>
-int launch_traverse(pn_grid& g, int queries, const F& f) {
-    const BinSet* Q = query_set(g, queries);
+int check_queries(pn_grid& g, int queries, const BinSet*& Q) {
+    Q = query_set(g, queries);
     if (!Q) {
         if (queries != PN_QUERIES_GRID && queries != PN_QUERIES_BINNED)
             return fail(PN_ERR_ARGUMENT, "queries must be PN_QUERIES_GRID or PN_QUERIES_BINNED");
         return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
     }
+    return PN_OK;
+}
+
+// Gathers the functor's per-neighbour fields into sorted order, then launches the
+// wave-per-cell traversal for the query set.
+template <int D, class TC, class TR, class F>
+int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* src) {
     if (Q->n_sel == 0) return PN_OK;
+    if constexpr (F::kFields > 0) {
+        const BinSet& G = g.grid;
+        const int64_t stride = std::max<int64_t>(G.cap, 1);
+        int rc = g.fields.ensure((size_t)F::kFields * stride * sizeof(TR), g.stream);
+        if (rc) return rc;
+        if (G.n_sel > 0) {
+            hipLaunchKernelGGL(pn::k_gather_fields<TR>, dim3(blocks_for(G.n_sel, 256)), dim3(256), 0, g.stream,
+                               G.sorted_idx.as<int32_t>(), G.status.as<pn::DevStatus>(), *src, g.fields.as<TR>(),
+                               stride);
+            PN_CHECK_LAUNCH();
+        }
+        f.fsorted = g.fields.p;
+        f.fstride = stride;
+    }
     // Upper bound on non-empty query cells; the kernel reads the exact count on device.
     int64_t nwork = std::min<int64_t>(g.geom.ncells, Q->n_sel);
     if (Q->stats_known) nwork = Q->stats.n_nonempty;
     if (nwork == 0) return PN_OK;
-    const int nblocks = (int)round_up8(nwork);
+    const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
+    const TR r = (TR)g.radius;
+    const TR r2 = r * r;
+    hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::value>), dim3(nblocks), dim3(pn::WAVE), 0, g.stream,
+                       g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+template <class F>
+int launch_traverse(pn_grid& g, int queries, const F& f) {
+    const BinSet* Q;
+    int rc = check_queries(g, queries, Q);
+    if (rc) return rc;
     return dispatch_trav(g, [&](auto dc, auto tcv, auto trv) -> int {
         constexpr int D = decltype(dc)::value;
         using TC = decltype(tcv);
         using TR = decltype(trv);
-        const TR r = (TR)g.radius;
-        const TR r2 = r * r;
-        hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC>::value>), dim3(nblocks), dim3(pn::WAVE), 0,
-                           g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
-        PN_CHECK_LAUNCH();
-        return PN_OK;
+        return launch_traverse_t<D, TC, TR, F>(g, Q, f, (const pn::FieldSrc<TR>*)nullptr);
     });
+}
+
+template <class TR>
+pn::FieldSrc<TR> fields(std::initializer_list<std::pair<const void*, int>> l) {
+    pn::FieldSrc<TR> s{};
+    s.n = 0;
+    for (auto& e : l) {
+        s.p[s.n] = (const TR*)e.first;
+        s.step[s.n] = e.second;
+        ++s.n;
+    }
+    return s;
 }
 
 }  // namespace
@@ -519,14 +585,19 @@ extern "C" {
 int pn_count_neighbors(pn_grid_t grid, int queries, int32_t* counts, int accumulate) {
     if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
     if (!counts) return fail(PN_ERR_ARGUMENT, "counts is NULL");
-    pn::CountF f{counts, accumulate};
+    pn::CountF f;
+    f.out = counts;
+    f.accumulate = accumulate;
     return launch_traverse(*grid, queries, f);
 }
 
 int pn_sum_neighbor_indices(pn_grid_t grid, int queries, int64_t* sums, int index_base, int accumulate) {
     if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
     if (!sums) return fail(PN_ERR_ARGUMENT, "sums is NULL");
-    pn::IndexSumF f{sums, accumulate, index_base};
+    pn::IndexSumF f;
+    f.out = sums;
+    f.accumulate = accumulate;
+    f.base = index_base;
     return launch_traverse(*grid, queries, f);
 }
 
@@ -550,7 +621,9 @@ int pn_csr_count(pn_grid_t grid, int queries, int64_t n_rows, int32_t* row_ptr, 
     hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(n_rows, 1 << 20), 256)), dim3(256), 0,
                        g.stream, g.csr_counts.as<int32_t>(), n_rows, 0);
     PN_CHECK_LAUNCH();
-    pn::CountF f{g.csr_counts.as<int32_t>(), 0};
+    pn::CountF f;
+    f.out = g.csr_counts.as<int32_t>();
+    f.accumulate = 0;
     if ((rc = launch_traverse(g, queries, f))) return rc;
     if ((rc = g.csr_scan.run(g.csr_counts.as<int32_t>(), n_rows, row_ptr, nullptr, st, g.stream))) return rc;
     PN_HIP(hipMemcpyAsync(g.csr_host_status, st, sizeof(pn::DevStatus), hipMemcpyDeviceToHost, g.stream));
@@ -575,7 +648,11 @@ int pn_csr_fill(pn_grid_t grid, int queries, int64_t n_rows, const int32_t* row_
     if (n_rows < Q->n_src) return fail(PN_ERR_ARGUMENT, "n_rows is smaller than the number of query points");
     if (g.csr_host_status && g.csr_host_status->max_count > pn::SORT_CAP)
         return fail(PN_ERR_UNSUPPORTED, "neighbor lists longer than 4096 entries are not supported");
-    pn::FillF f{row_ptr, col_idx, index_base};
+    pn::FillF f;
+    f.row_ptr = row_ptr;
+    f.col = col_idx;
+    f.base = index_base;
+    f.row_end = 0;
     int rc = launch_traverse(g, queries, f);
     if (rc) return rc;
     if (n_rows == 0) return PN_OK;
@@ -613,6 +690,299 @@ int pn_list_sum_neighbor_indices(const int32_t* row_ptr, const int32_t* col_idx,
     hipLaunchKernelGGL(pn::k_list_index_sum, dim3(nblocks), dim3(pn::WAVE), 0, (hipStream_t)stream, row_ptr,
                        col_idx, sel_first, sel_list, sel_count, (long long)index_base - stored_base, sums,
                        accumulate, nblocks);
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Interaction functors: SPH density / pressure acceleration, pair energy, adjoints.
+// ------------------------------------------------------------------------------------
+namespace {
+
+int sph_precheck(pn_grid_t grid, int kernel, double h) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (grid->nd < 2) return fail(PN_ERR_UNSUPPORTED, "the SPH kernels are defined for 2D and 3D");
+    if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
+        return fail(PN_ERR_ARGUMENT, "unknown smoothing kernel");
+    if (!(h > 0.0)) return fail(PN_ERR_ARGUMENT, "smoothing length must be positive");
+    return PN_OK;
+}
+
+// Dispatch over (D in {2,3}, TC, TR) for the physics functors.
+template <class Fn>
+int dispatch_phys(const pn_grid& g, Fn&& fn) {
+    return dispatch_trav(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        if constexpr (D == 1) {
+            return fail(PN_ERR_UNSUPPORTED, "the interaction functors are defined for 2D and 3D");
+        } else {
+            return fn(dc, tcv, trv);
+        }
+    });
+}
+
+int require_self(pn_grid& g, int queries) {
+    if (queries != PN_QUERIES_GRID)
+        return fail(PN_ERR_UNSUPPORTED, "this functor needs x == y (the grid's own points)");
+    if (!g.grid.valid) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
+    return PN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn_sph_density(pn_grid_t grid, int queries, int kernel, double h, const void* mass, void* rho, int accumulate) {
+    int rc = sph_precheck(grid, kernel, h);
+    if (rc) return rc;
+    if (!mass || !rho) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    pn_grid& g = *grid;
+    const BinSet* Q;
+    if ((rc = check_queries(g, queries, Q))) return rc;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::DensityF<TR, D> f;
+        f.kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        f.rho = (TR*)rho;
+        f.accumulate = accumulate;
+        auto src = fields<TR>({{mass, 1}});
+        return launch_traverse_t<D, TC, TR>(g, Q, f, &src);
+    });
+}
+
+int pn_sph_pressure_accel(pn_grid_t grid, int kernel, double h, const void* mass, const void* P, void* dv,
+                          int accumulate) {
+    int rc = sph_precheck(grid, kernel, h);
+    if (rc) return rc;
+    if (!mass || !P || !dv) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    pn_grid& g = *grid;
+    if ((rc = require_self(g, PN_QUERIES_GRID))) return rc;
+    const BinSet* Q = &g.grid;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::PressureAccelF<TR, D> f;
+        f.kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        f.P = (const TR*)P;
+        f.dv = (TR*)dv;
+        f.accumulate = accumulate;
+        auto src = fields<TR>({{mass, 1}, {P, 1}});
+        return launch_traverse_t<D, TC, TR>(g, Q, f, &src);
+    });
+}
+
+int pn_sph_density_vjp(pn_grid_t grid, int kernel, double h, const void* mass, const void* rho_bar, void* x_bar,
+                       void* mass_bar, int accumulate) {
+    int rc = sph_precheck(grid, kernel, h);
+    if (rc) return rc;
+    if (!mass || !rho_bar || !x_bar) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    pn_grid& g = *grid;
+    if ((rc = require_self(g, PN_QUERIES_GRID))) return rc;
+    const BinSet* Q = &g.grid;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::DensityVjpF<TR, D> f;
+        f.kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        f.mass = (const TR*)mass;
+        f.g = (const TR*)rho_bar;
+        f.xbar = (TR*)x_bar;
+        f.mbar = (TR*)mass_bar;
+        f.accumulate = accumulate;
+        auto src = fields<TR>({{mass, 1}, {rho_bar, 1}});
+        return launch_traverse_t<D, TC, TR>(g, Q, f, &src);
+    });
+}
+
+int pn_sph_pressure_accel_vjp(pn_grid_t grid, int kernel, double h, const void* mass, const void* P,
+                              const void* dv_bar, void* x_bar, void* P_bar, void* mass_bar, int accumulate) {
+    int rc = sph_precheck(grid, kernel, h);
+    if (rc) return rc;
+    if (!mass || !P || !dv_bar || !x_bar) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    pn_grid& g = *grid;
+    if ((rc = require_self(g, PN_QUERIES_GRID))) return rc;
+    const BinSet* Q = &g.grid;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::PressureVjpF<TR, D> f;
+        f.kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        f.mass = (const TR*)mass;
+        f.P = (const TR*)P;
+        f.abar = (const TR*)dv_bar;
+        f.xbar = (TR*)x_bar;
+        f.Pbar = (TR*)P_bar;
+        f.mbar = (TR*)mass_bar;
+        f.accumulate = accumulate;
+        const TR* ab = (const TR*)dv_bar;
+        auto src = D == 3 ? fields<TR>({{mass, 1}, {P, 1}, {ab, 3}, {ab + 1, 3}, {ab + 2, 3}})
+                          : fields<TR>({{mass, 1}, {P, 1}, {ab, 2}, {ab + 1, 2}});
+        return launch_traverse_t<D, TC, TR>(g, Q, f, &src);
+    });
+}
+
+int pn_pair_energy(pn_grid_t grid, int queries, double eps, double cutoff, void* e, int accumulate) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!e) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (!(cutoff > 0.0)) return fail(PN_ERR_ARGUMENT, "cutoff must be positive");
+    pn_grid& g = *grid;
+    const BinSet* Q;
+    int rc = check_queries(g, queries, Q);
+    if (rc) return rc;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::EnergyF<TR, D> f;
+        f.eps = (TR)eps;
+        f.inv_rc = (TR)(1.0 / cutoff);
+        f.e = (TR*)e;
+        f.accumulate = accumulate;
+        return launch_traverse_t<D, TC, TR>(g, Q, f, (const pn::FieldSrc<TR>*)nullptr);
+    });
+}
+
+int pn_pair_energy_vjp(pn_grid_t grid, double eps, double cutoff, const void* e_bar, void* x_bar, int accumulate) {
+    if (!grid) return fail(PN_ERR_ARGUMENT, "grid is NULL");
+    if (!e_bar || !x_bar) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (!(cutoff > 0.0)) return fail(PN_ERR_ARGUMENT, "cutoff must be positive");
+    pn_grid& g = *grid;
+    int rc;
+    if ((rc = require_self(g, PN_QUERIES_GRID))) return rc;
+    const BinSet* Q = &g.grid;
+    return dispatch_phys(g, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        pn::EnergyVjpF<TR, D> f;
+        f.eps = (TR)eps;
+        f.inv_rc = (TR)(1.0 / cutoff);
+        f.wq = (const TR*)e_bar;
+        f.xbar = (TR*)x_bar;
+        f.accumulate = accumulate;
+        auto src = fields<TR>({{e_bar, 1}});
+        return launch_traverse_t<D, TC, TR>(g, Q, f, &src);
+    });
+}
+
+int pn_tait_eos(const void* rho, void* p, int64_t n, int dtype, double rho0, double c, double gamma, void* stream) {
+    if (!rho || !p) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (n <= 0) return PN_OK;
+    if (dtype == PN_FLOAT32) {
+        hipLaunchKernelGGL(pn::k_tait_eos<float>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const float*)rho, (float*)p, n, (float)rho0, (float)c, (float)gamma);
+    } else if (dtype == PN_FLOAT64) {
+        hipLaunchKernelGGL(pn::k_tait_eos<double>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const double*)rho, (double*)p, n, rho0, c, gamma);
+    } else {
+        return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    }
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Batches of independent clouds, tuning, synthetic displacement.
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype, int64_t n_clouds,
+                    const double* min_corners, const int64_t* n_cells, double search_radius,
+                    int32_t max_points_per_cell, const int64_t* point_offsets, pn_grid_t* out) {
+    if (!out) return fail(PN_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (n_clouds < 1 || n_clouds > INT32_MAX / 2) return fail(PN_ERR_ARGUMENT, "n_clouds must be positive");
+    if (!min_corners || !n_cells || !point_offsets) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (point_offsets[0] != 0) return fail(PN_ERR_ARGUMENT, "point_offsets must start at 0");
+    std::vector<pn::GeomEntry> table((size_t)n_clouds);
+    std::vector<int32_t> cell_offsets((size_t)n_clouds + 1);
+    int64_t C = 0;
+    for (int64_t c = 0; c < n_clouds; ++c) {
+        if (point_offsets[c + 1] < point_offsets[c]) return fail(PN_ERR_ARGUMENT, "point_offsets must not decrease");
+        int64_t cc = 1;
+        for (int d = 0; d < 3; ++d) {
+            const int64_t nd_ = d < ndims ? n_cells[c * ndims + d] : 1;
+            if (d < ndims && nd_ < 3) return fail(PN_ERR_ARGUMENT, "every dimension needs at least 3 cells");
+            table[c].n[d] = (int)nd_;
+            table[c].minc[d] = d < ndims ? min_corners[c * ndims + d] : 0.0;
+            cc *= nd_;
+        }
+        table[c].cell_off = (int)C;
+        cell_offsets[c] = (int32_t)C;
+        C += cc;
+        if (C > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "more than 2^31 - 2 cells are not supported");
+    }
+    cell_offsets[n_clouds] = (int32_t)C;
+    const int64_t n_points = point_offsets[n_clouds];
+    if (n_points > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "more than 2^31 - 2 points are not supported");
+    // a 3-cell dummy geometry satisfies the single-grid checks; the table overrides it
+    int64_t dummy[3] = {3, 3, 3};
+    double zero[3] = {0, 0, 0};
+    pn_grid_t g;
+    int rc = pn_grid_create(ndims, coord_dtype, radius_dtype, cell_dtype, zero, dummy, search_radius,
+                            max_points_per_cell, &g);
+    if (rc) return rc;
+    g->geom.ncells = (int)C;
+    std::vector<int32_t> cloud_of((size_t)std::max<int64_t>(n_points, 1));
+    for (int64_t c = 0; c < n_clouds; ++c)
+        for (int64_t p = point_offsets[c]; p < point_offsets[c + 1]; ++p) cloud_of[p] = (int32_t)c;
+    auto upload = [&](Buf& b, const void* src, size_t bytes) -> int {
+        int r = b.ensure(bytes, nullptr);
+        if (r) return r;
+        PN_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+        return PN_OK;
+    };
+    if ((rc = upload(g->geom_table, table.data(), table.size() * sizeof(pn::GeomEntry))) ||
+        (rc = upload(g->cell_offs, cell_offsets.data(), cell_offsets.size() * 4)) ||
+        (rc = upload(g->cloud_of, cloud_of.data(), cloud_of.size() * 4))) {
+        delete g;
+        return rc;
+    }
+    g->geom.table = g->geom_table.as<pn::GeomEntry>();
+    g->geom.cell_offsets = g->cell_offs.as<int32_t>();
+    g->geom.cloud_of_point = g->cloud_of.as<int32_t>();
+    g->geom.n_clouds = (int)n_clouds;
+    g->host_table = table;
+    g->host_point_offsets.assign(point_offsets, point_offsets + n_clouds + 1);
+    *out = g;
+    return PN_OK;
+}
+
+int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value) {
+    if (!grid || !key) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    const std::string k(key);
+    if (k == "persist_blocks") {
+        if (value < 8 || value % 8) return fail(PN_ERR_ARGUMENT, "persist_blocks must be a positive multiple of 8");
+        grid->persist_blocks = value;
+        return PN_OK;
+    }
+    return fail(PN_ERR_ARGUMENT, "unknown tuning key " + k);
+}
+
+int pn_displace_uniform(void* x, int64_t n, int dtype, double amplitude, uint64_t seed, int64_t* step,
+                        void* stream) {
+    if (!x || !step) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (n <= 0) return PN_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == PN_FLOAT32)
+        hipLaunchKernelGGL(pn::k_displace<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float*)x, n,
+                           (float)amplitude, seed, (const int64_t*)step);
+    else if (dtype == PN_FLOAT64)
+        hipLaunchKernelGGL(pn::k_displace<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (double*)x, n,
+                           amplitude, seed, (const int64_t*)step);
+    else
+        return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    PN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(pn::k_step_increment, dim3(1), dim3(1), 0, s, step);
     PN_CHECK_LAUNCH();
     return PN_OK;
 }

@@ -1,17 +1,11 @@
-// Device kernels of the MI355X neighbourhood-search engine (gfx950, wave64).
-//
-//  Build (initialize!/update!, ParallelUpdate, src/nhs_grid.jl:255-281):
-//    counting sort of the selected points by linear cell index
+// Build kernels of the MI355X neighbourhood-search engine (gfx950, wave64):
+// initialize!/update! with ParallelUpdate semantics (src/nhs_grid.jl:255-281) as a
+// counting sort of the selected points by linear cell index.
 //      k_bin       cell key per point + bounds check + wave-aggregated histogram
 //      k_scan_*    exclusive scan of the histogram -> cell_start, non-empty cell list
 //      k_scatter   point -> provisional slot in its cell segment
 //      k_canon     deterministic order inside each cell (original index ascending)
 //                  and the cell-sorted coordinate planes
-//  Traversal (foreach_point_neighbor, hot loop src/nhs_grid.jl:519-575):
-//      k_traverse  one wavefront per query cell; the 3^D stencil is 3^(D-1) contiguous
-//                  x-rows of the sorted order, staged in LDS with coalesced loads; lanes
-//                  split as (query, candidate slice)
-//  Precomputed lists (src/nhs_precomputed.jl:187-206): count, scan, fill, sort rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,14 +16,39 @@ namespace pn {
 
 constexpr int WAVE = 64;
 
-// Grid geometry.  minc and cs are exact in the cell-computation type.
+// Geometry of one cloud's grid inside a batch: its cells occupy the linear range
+// [cell_off, cell_off + n0*n1*n2) of the concatenated cell table.
+struct GeomEntry {
+    int n[3];
+    int cell_off;
+    double minc[3];
+};
+
+// Grid geometry.  minc and cs are exact in the cell-computation type.  A batch of
+// independent clouds (same radius, own boxes) sets `table`: point p belongs to cloud
+// cloud_of_point[p], whose cells start at cell_offsets[cloud].
 struct Geom {
     int nd;
     int n[3];      // cells per dimension, 1 for unused dimensions
-    int ncells;
+    int ncells;    // total cells (all clouds)
     double minc[3];
     double cs;
+    const GeomEntry* table;
+    const int32_t* cloud_of_point;
+    const int32_t* cell_offsets;  // n_clouds + 1
+    int n_clouds;
 };
+
+// Cloud of a linear cell index in a batch (binary search over cell_offsets).
+__device__ __forceinline__ int cloud_of_cell(const Geom& g, int c) {
+    int lo = 0, hi = g.n_clouds;  // cell_offsets[lo] <= c < cell_offsets[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (g.cell_offsets[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // Device-side status of one binning pass.
 struct DevStatus {
@@ -82,17 +101,18 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
 // condition reads 1 <= f <= n_d - 2; NaN and +-Inf fail both comparisons, matching the
 // saturating floor_to_int (src/util.jl:19-34) whose saturated values are out of range.
 template <int D, class TC, class TCELL>
-__device__ __forceinline__ int cell_key(const Geom& g, const TC* __restrict__ xp, bool& ok) {
+__device__ __forceinline__ int cell_key(const int (&n)[3], const double (&minc)[3], double cs,
+                                        const TC* __restrict__ xp, bool& ok) {
     int key = 0, stride = 1;
     ok = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        TCELL q = ((TCELL)xp[d] - (TCELL)g.minc[d]) / (TCELL)g.cs;
+        TCELL q = ((TCELL)xp[d] - (TCELL)minc[d]) / (TCELL)cs;
         TCELL f = floor(q);
-        bool in = (f >= (TCELL)1) && (f <= (TCELL)(g.n[d] - 2));
+        bool in = (f >= (TCELL)1) && (f <= (TCELL)(n[d] - 2));
         ok = ok && in;
         key += (in ? (int)f : 0) * stride;
-        stride *= g.n[d];
+        stride *= n[d];
     }
     return key;
 }
@@ -115,7 +135,13 @@ __global__ __launch_bounds__(256) void k_bin(Geom g, const TC* __restrict__ y, i
     if (a < n_sel) {
         int64_t p = sel_point(a, sel_first, sel_list);
         bool ok;
-        int k = cell_key<D, TC, TCELL>(g, y + (int64_t)D * p, ok);
+        int k;
+        if (g.table) {
+            const GeomEntry e = g.table[g.cloud_of_point[p]];
+            k = e.cell_off + cell_key<D, TC, TCELL>(e.n, e.minc, g.cs, y + (int64_t)D * p, ok);
+        } else {
+            k = cell_key<D, TC, TCELL>(g.n, g.minc, g.cs, y + (int64_t)D * p, ok);
+        }
         if (ok) key = k;
         else atomicMin(&st->first_bad, (unsigned long long)a);
         keys[a] = key;
@@ -305,273 +331,27 @@ __global__ __launch_bounds__(256) void k_regather(const int32_t* __restrict__ so
     for (int d = 0; d < D; ++d) coords[(int64_t)d * cap + k] = y[(int64_t)D * p + d];
 }
 
-// ------------------------------------------------------------------------------------
-// Traversal
-// ------------------------------------------------------------------------------------
-
-template <int D>
-struct Rows {
-    static constexpr int R = D == 1 ? 1 : (D == 2 ? 3 : 9);
-};
-
-// Functor contract (fixed menu; arbitrary closures cannot cross the C-ABI):
-//   Acc                 per-query accumulator, combined over candidate slices
-//   kIndex              stage original neighbour indices in LDS
-//   init() / combine()  accumulator identity and merge
-//   pair(acc, pd, d2, j, scratch)   called for every accepted pair (d2 <= r^2)
-//   write(i, acc)       store for original query index i
-struct CountF {
-    using Acc = int;
-    static constexpr bool kIndex = false;
-    int32_t* out;
-    int accumulate;
-    __device__ Acc init() const { return 0; }
-    template <class TR, int D>
-    __device__ void pair(Acc& a, const TR (&)[D], TR, int, int*) const { a += 1; }
-    __device__ void combine(Acc& a, const Acc& b) const { a += b; }
-    __device__ void write(int64_t i, const Acc& a) const { out[i] = accumulate ? out[i] + a : a; }
-};
-
-struct IndexSumF {
-    using Acc = long long;
-    static constexpr bool kIndex = true;
-    int64_t* out;
-    int accumulate;
-    int base;
-    __device__ Acc init() const { return 0; }
-    template <class TR, int D>
-    __device__ void pair(Acc& a, const TR (&)[D], TR, int j, int*) const { a += (long long)j + base; }
-    __device__ void combine(Acc& a, const Acc& b) const { a += b; }
-    __device__ void write(int64_t i, const Acc& a) const { out[i] = accumulate ? out[i] + a : a; }
-};
-
-// CSR fill: accepted neighbours go to row i at positions claimed with an LDS counter
-// per query; the rows are sorted afterwards, so the claim order does not matter.
-struct FillF {
-    using Acc = int;
-    static constexpr bool kIndex = true;
-    const int32_t* row_ptr;
-    int32_t* col;
-    int base;
-    int row_end;  // set per query by the traversal; guards the row against a stale count
-    __device__ Acc init() const { return 0; }
-    __device__ int row_begin(int64_t i) const { return row_ptr[i]; }
-    template <class TR, int D>
-    __device__ void pair(Acc&, const TR (&)[D], TR, int j, int* qcnt) const {
-        int slot = atomicAdd(qcnt, 1);
-        if (slot < row_end) col[slot] = j + base;
-    }
-    __device__ void combine(Acc&, const Acc&) const {}
-    __device__ void write(int64_t, const Acc&) const {}
-};
-
-template <class F>
-struct IsFill {
-    static constexpr bool value = false;
-};
-template <>
-struct IsFill<FillF> {
-    static constexpr bool value = true;
-};
-
-// One wavefront (= one workgroup) per non-empty query cell.
-//  1. lanes 0..R-1 read the [begin, end) of their stencil row (3 x-adjacent cells)
-//  2. candidates are staged in LDS in chunks of CH, coalesced; a sentinel at +Inf pads
-//     the chunk to S*L entries so every lane runs the same trip count
-//  3. the nq query points of the cell (up to 64 at a time) take S = 64 / nq slices of
-//     the chunk each; slice s covers [s*L, s*L + L) with L odd (conflict-free banks)
-//  4. slice partials are combined in LDS in slice order (deterministic) and written
-//     to the query's original index.
-template <int D, class TC, class TR, class F, int CH>
-__global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G, TR r2, F f, int nblocks) {
-    constexpr int R = Rows<D>::R;
-    // padded = S * L with L = ceil(mc / S) | 1 <= mc + 2S - 1 <= CH + 127
-    constexpr int CAP = CH + 2 * WAVE;
-    __shared__ TC s_c[D][CAP];
-    __shared__ int s_j[F::kIndex ? CAP : 1];
-    __shared__ typename F::Acc s_red[WAVE];
-    __shared__ int s_cnt[IsFill<F>::value ? WAVE : 1];
-    __shared__ int s_end[IsFill<F>::value ? WAVE : 1];
-
-    const int lane = threadIdx.x;
-    const int w = xcd_remap(blockIdx.x, nblocks);
-    if (w >= Q.st->n_nonempty) return;  // uniform over the workgroup (one wavefront)
-    const int qc = Q.nonempty[w];
-    const int qs = Q.start[qc], qe = Q.start[qc + 1];
-
-    // stencil rows
-    int rb = 0, rlen = 0;
-    if (lane < R) {
-        int off = 0;
-        if (D >= 2) off += ((lane % 3) - 1) * g.n[0];
-        if (D >= 3) off += ((lane / 3) - 1) * g.n[0] * g.n[1];
-        const int base = qc - 1 + off;
-        rb = G.start[base];
-        rlen = G.start[base + 3] - rb;
-    }
-    int rbeg[R], pre[R + 1];
-    pre[0] = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        rbeg[r] = __builtin_amdgcn_readlane(rb, r);
-        pre[r + 1] = pre[r] + __builtin_amdgcn_readlane(rlen, r);
-    }
-    const int M = pre[R];
-
-    const TC* __restrict__ gc = (const TC*)G.coords;
-    const TC* __restrict__ qcrd = (const TC*)Q.coords;
-
-    for (int q0 = qs; q0 < qe; q0 += WAVE) {
-        const int nq = min(WAVE, qe - q0);
-        const int S = WAVE / nq;
-        const int q = lane % nq, s = lane / nq;
-        const bool act = s < S;
-        TC xi[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) xi[d] = act ? qcrd[(int64_t)d * Q.cap + q0 + q] : (TC)0;
-        typename F::Acc acc = f.init();
-        const int64_t orig_i = act ? Q.sorted_idx[q0 + q] : 0;
-        __syncthreads();
-        if constexpr (IsFill<F>::value) {
-            if (lane < nq) {
-                s_cnt[lane] = f.row_begin(orig_i);
-                s_end[lane] = f.row_begin(orig_i + 1);
-            }
-        }
-        int* qcnt = nullptr;
-
-        for (int m0 = 0; m0 < M; m0 += CH) {
-            const int mc = min(CH, M - m0);
-            int L = (mc + S - 1) / S;
-            L |= 1;
-            const int padded = S * L;
-            __syncthreads();
-            for (int k = lane; k < padded; k += WAVE) {
-                if (k < mc) {
-                    const int gk = m0 + k;
-                    int r = 0;
-#pragma unroll
-                    for (int t = 1; t < R; ++t) r += (gk >= pre[t]) ? 1 : 0;
-                    int rsel_beg = rbeg[0], rsel_pre = pre[0];
-#pragma unroll
-                    for (int t = 1; t < R; ++t)
-                        if (r == t) {
-                            rsel_beg = rbeg[t];
-                            rsel_pre = pre[t];
-                        }
-                    const int src = rsel_beg + (gk - rsel_pre);
-#pragma unroll
-                    for (int d = 0; d < D; ++d) s_c[d][k] = gc[(int64_t)d * G.cap + src];
-                    if constexpr (F::kIndex) s_j[k] = G.sorted_idx[src];
-                } else {
-#pragma unroll
-                    for (int d = 0; d < D; ++d) s_c[d][k] = (TC)INFINITY;
-                    if constexpr (F::kIndex) s_j[k] = 0;
-                }
-            }
-            __syncthreads();
-            if (act) {
-                if constexpr (IsFill<F>::value) {
-                    qcnt = &s_cnt[q];
-                    f.row_end = s_end[q];
-                }
-                const int kb = s * L;
-#pragma unroll 4
-                for (int t = 0; t < L; ++t) {
-                    const int k = kb + t;
-                    TR pd[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) pd[d] = (TR)(xi[d] - s_c[d][k]);
-                    TR d2 = pd[0] * pd[0];
-#pragma unroll
-                    for (int d = 1; d < D; ++d) {
-                        TR p2 = pd[d] * pd[d];
-                        d2 = d2 + p2;
-                    }
-                    if (d2 <= r2) {
-                        int j = 0;
-                        if constexpr (F::kIndex) j = s_j[k];
-                        f.template pair<TR, D>(acc, pd, d2, j, qcnt);
-                    }
-                }
-            }
-        }
-        if constexpr (!IsFill<F>::value) {
-            __syncthreads();
-            s_red[lane] = acc;
-            __syncthreads();
-            if (act && s == 0) {
-                for (int t = 1; t < S; ++t) f.combine(acc, s_red[q + t * nq]);
-                f.write(orig_i, acc);
-            }
-        }
-    }
+// Synthetic per-step displacement U[-a, a] of every coordinate, a pure function of
+// (seed, step, index): SplitMix64 of a counter, top 53 bits.  The step counter lives on
+// the device so a captured graph advances it on every replay (the perturbation of
+// benchmarks/update.jl:28-49, made replayable).
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
 }
 
-// Bitonic sort of each CSR row in LDS, one wavefront per row, rows up to SORT_CAP.
-constexpr int SORT_CAP = 4096;
-
-__global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ row_ptr, int64_t n_rows,
-                                                    int32_t* __restrict__ col, int nblocks) {
-    __shared__ int32_t s[SORT_CAP];
-    const int lane = threadIdx.x;
-    const int64_t row = xcd_remap(blockIdx.x, nblocks);
-    if (row >= n_rows) return;
-    const int b = row_ptr[row], e = row_ptr[row + 1];
-    const int n = e - b;
-    if (n <= 1 || n > SORT_CAP) return;
-    int P = 1;
-    while (P < n) P <<= 1;
-    for (int k = lane; k < P; k += WAVE) s[k] = k < n ? col[b + k] : INT32_MAX;
-    __syncthreads();
-    for (int size = 2; size <= P; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < (P >> 1); t += WAVE) {
-                const int lo = 2 * t - (t & (stride - 1));
-                const int hi = lo + stride;
-                const bool up = (lo & size) == 0;
-                const int32_t a = s[lo], c = s[hi];
-                if ((a > c) == up) {
-                    s[lo] = c;
-                    s[hi] = a;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = lane; k < n; k += WAVE) col[b + k] = s[k];
+template <class T>
+__global__ __launch_bounds__(256) void k_displace(T* __restrict__ x, int64_t n, T a, unsigned long long seed,
+                                                  const int64_t* __restrict__ step) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = splitmix64(seed ^ splitmix64((unsigned long long)*step));
+    const double u = (double)(splitmix64(key + (unsigned long long)i) >> 11) * (1.0 / 9007199254740992.0);
+    x[i] += (T)((2.0 * u - 1.0) * (double)a);
 }
 
-// ------------------------------------------------------------------------------------
-// Precomputed-list traversal (src/nhs_precomputed.jl:210-247): iterate the stored list
-// of each selected row; no radius test.
-// ------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_list_count(const int32_t* __restrict__ row_ptr, int64_t sel_first,
-                                                    const int32_t* __restrict__ sel_list, int64_t n_sel,
-                                                    int32_t* __restrict__ out, int accumulate) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_sel) return;
-    const int64_t i = sel_point(a, sel_first, sel_list);
-    const int32_t c = row_ptr[i + 1] - row_ptr[i];
-    out[i] = accumulate ? out[i] + c : c;
-}
-
-// Sum of (stored value - stored_base + index_base) over each selected row; one
-// wavefront per row, coalesced over the row.
-__global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* __restrict__ row_ptr,
-                                                         const int32_t* __restrict__ col, int64_t sel_first,
-                                                         const int32_t* __restrict__ sel_list, int64_t n_sel,
-                                                         long long shift, int64_t* __restrict__ out,
-                                                         int accumulate, int nblocks) {
-    const int64_t a = xcd_remap(blockIdx.x, nblocks);
-    if (a >= n_sel) return;
-    const int64_t i = sel_point(a, sel_first, sel_list);
-    const int b = row_ptr[i], e = row_ptr[i + 1];
-    long long s = 0;
-    for (int k = b + (int)threadIdx.x; k < e; k += WAVE) s += (long long)col[k] + shift;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
-    if (threadIdx.x == 0) out[i] = accumulate ? out[i] + s : s;
-}
+__global__ void k_step_increment(int64_t* step) { *step += 1; }
 
 }  // namespace pn

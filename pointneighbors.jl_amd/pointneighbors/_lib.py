@@ -74,7 +74,21 @@ SIGNATURES = {
     "pn_list_count_neighbors": (_c_int, [_vp, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_int, _vp]),
     "pn_list_sum_neighbor_indices": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_i64, _c_i64, _vp, _vp,
                                               _c_int, _c_int, _vp]),
+    "pn_sph_density": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _vp, _vp, _c_int]),
+    "pn_sph_density_vjp": (_c_int, [_vp, _c_int, _c_dbl, _vp, _vp, _vp, _vp, _c_int]),
+    "pn_sph_pressure_accel": (_c_int, [_vp, _c_int, _c_dbl, _vp, _vp, _vp, _c_int]),
+    "pn_sph_pressure_accel_vjp": (_c_int, [_vp, _c_int, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp, _c_int]),
+    "pn_pair_energy": (_c_int, [_vp, _c_int, _c_dbl, _c_dbl, _vp, _c_int]),
+    "pn_pair_energy_vjp": (_c_int, [_vp, _c_dbl, _c_dbl, _vp, _vp, _c_int]),
+    "pn_tait_eos": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl, _vp]),
+    "pn_batch_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_i64, _p_dbl, _p_i64, _c_dbl, _c_i32,
+                                 _p_i64, ctypes.POINTER(_vp)]),
+    "pn_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_i64]),
+    "pn_displace_uniform": (_c_int, [_vp, _c_i64, _c_int, _c_dbl, ctypes.c_uint64, _vp, _vp]),
 }
+
+PN_KERNEL_CUBIC_SPLINE = 0
+PN_KERNEL_WENDLAND_C2 = 1
 
 _lib = None
 

@@ -160,10 +160,10 @@ def initialize(search, x, y, *, parallelization_backend=None, eachindex_y=None):
 
 
 def update(search, x, y, *, points_moving=(True, True), parallelization_backend=None,
-           eachindex_y=None):
+           eachindex_y=None, **kw):
     """update!(search, x, y; points_moving, eachindex_y) (src/nhs_grid.jl:283-292,
     src/nhs_precomputed.jl:149-169)."""
-    return search.update(x, y, points_moving=points_moving, eachindex_y=eachindex_y)
+    return search.update(x, y, points_moving=points_moving, eachindex_y=eachindex_y, **kw)
 
 
 def requires_update(search):

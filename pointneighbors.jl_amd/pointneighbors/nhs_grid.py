@@ -176,7 +176,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         return tuple(out)
 
     # -- build -----------------------------------------------------------------------
-    def _build(self, y, eachindex_y):
+    def _build(self, y, eachindex_y, check_errors=True):
         nd = self.ndims
         y = _device.as_coordinates(y, nd, "neighbor coordinates")
         self._built = False
@@ -193,8 +193,9 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         first, count, ptr, keep = _device.selection(eachindex_y, n, y.device)
         self._set_stream(y)
         lib = load()
-        st = lib.pn_grid_update(self._handle, ctypes.c_void_p(y.data_ptr()), n, first, count,
-                                ctypes.c_void_p(ptr) if ptr else None)
+        fn = lib.pn_grid_update if check_errors else lib.pn_grid_update_async
+        st = fn(self._handle, ctypes.c_void_p(y.data_ptr()), n, first, count,
+                ctypes.c_void_p(ptr) if ptr else None)
         if st in (PN_ERR_OUT_OF_BOUNDS,):
             p = ctypes.c_int64()
             cell = (ctypes.c_int64 * nd)()
@@ -213,13 +214,18 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self._build(y, eachindex_y)
         return self
 
-    def update(self, x, y, *, points_moving=(True, True), eachindex_y=None):
+    def update(self, x, y, *, points_moving=(True, True), eachindex_y=None, check_errors=True):
         """update!(nhs, x, y; points_moving, eachindex_y) (src/nhs_grid.jl:283-292): only
-        y's motion matters; the update is a full re-binning (:470-477)."""
+        y's motion matters; the update is a full re-binning (:470-477).  check_errors=False
+        only enqueues the rebuild (no synchronization); check() raises its errors later."""
         if not points_moving[1]:
             return self
-        self._build(y, eachindex_y)
+        self._build(y, eachindex_y, check_errors)
         return self
+
+    def check(self):
+        """Raises the errors of the last update(check_errors=False)."""
+        check(load().pn_grid_check(self._handle), "update!")
 
     def copy(self, search_radius, n_points):
         return copy_grid_search(self, search_radius, n_points)

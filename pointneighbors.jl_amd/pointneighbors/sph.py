@@ -1,10 +1,62 @@
-"""Kernel functors beyond counting: the base class the traversal dispatches on.
+"""Interaction functors that run as dedicated HIP kernels, and their differentiable
+wrappers.
 
-The SPH density / pressure-force and pair-energy functors and their adjoints subclass
-KernelFunctor and implement `run_grid` (grid traversal) and `run_lists`
-(precomputed-list traversal)."""
+Functors (the `f` of foreach_point_neighbor; outputs indexed by original point index):
+  SPHDensity(rho, mass, h)                rho[i] += m[j] W(d)            (self included)
+  SPHPressureAcceleration(dv, mass, P, h) dv[:, i] -= m[j] (P[i] + P[j]) grad W_ij, j != i
+  WendlandPairEnergy(e, eps, cutoff)      e[i] += 1/2 phi(d), j != i
+
+Differentiable functions (torch.autograd.Function: the counterpart of the
+ChainRulesCore.rrule a Julia binding would define).  Their backward passes call
+hand-written adjoint kernels that recompute the traversal; neighbour sets are held fixed
+(piecewise constant in the coordinates), so gradients are those of the fixed-pair sums:
+  sph_density(x, mass, nhs, h)             -> rho
+  sph_pressure_acceleration(x, mass, P, nhs, h) -> dv (D x N)
+  pair_energy(x, nhs, eps, cutoff)          -> e (per point; E = e.sum())
+`nhs` must have been updated with x (x is the neighbour set and the query set).
+
+The physics is not part of the reference (it lives in TrixiParticles.jl); the formulas
+are the textbook ones listed in pointneighbors.jl_amd/csrc/pn_sph.h.
+"""
 
 from __future__ import annotations
+
+import ctypes
+
+from . import _device
+from ._lib import (PN_KERNEL_CUBIC_SPLINE, PN_KERNEL_WENDLAND_C2, PN_QUERIES_GRID, ArgumentError,
+                   PointNeighborsError, check, load)
+
+KERNELS = {"cubic_spline": PN_KERNEL_CUBIC_SPLINE, "wendland_c2": PN_KERNEL_WENDLAND_C2}
+
+
+def _kernel_code(kernel):
+    if kernel not in KERNELS:
+        raise ArgumentError(f"unknown smoothing kernel {kernel!r}; use one of {sorted(KERNELS)}")
+    return KERNELS[kernel]
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _field(t, n, dtype, name, ndims=None):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise ArgumentError(f"{name} must be a GPU tensor")
+    if t.dtype != dtype:
+        raise ArgumentError(f"{name} must have the radius type {dtype}")
+    if ndims is None:
+        if t.dim() != 1 or t.shape[0] != n or not t.is_contiguous():
+            raise ArgumentError(f"{name} must be a contiguous vector of {n} entries")
+    else:
+        if tuple(t.shape) != (ndims, n) or (n > 1 and t.stride() != (1, ndims)):
+            raise ArgumentError(f"{name} must be a {ndims} x {n} matrix with strides (1, {ndims})")
+
+
+def _radius_torch_dtype(nhs):
+    import torch
+    return torch.float32 if nhs.radius_dtype == "float32" else torch.float64
 
 
 class KernelFunctor:
@@ -13,5 +65,179 @@ class KernelFunctor:
     def run_grid(self, nhs, x, y, points):  # pragma: no cover - abstract
         raise NotImplementedError
 
-    def run_lists(self, nhs, x, y, points):  # pragma: no cover - abstract
-        raise NotImplementedError
+    def run_lists(self, nhs, x, y, points):
+        raise ArgumentError(f"{type(self).__name__} is not available on precomputed lists")
+
+
+class SPHDensity(KernelFunctor):
+    def __init__(self, rho, mass, smoothing_length, kernel="cubic_spline", accumulate=True):
+        self.rho, self.mass, self.h = rho, mass, float(smoothing_length)
+        self.kernel = _kernel_code(kernel)
+        self.accumulate = accumulate
+
+    def run_grid(self, nhs, x, y, points):
+        mode, x = nhs._prepare(x, y, points)
+        td = _radius_torch_dtype(nhs)
+        _field(self.rho, x.shape[1], td, "rho")
+        _field(self.mass, y.shape[1], td, "mass")
+        check(load().pn_sph_density(nhs._handle, mode, self.kernel, self.h, _vp(self.mass), _vp(self.rho),
+                                    1 if self.accumulate else 0), "SPHDensity")
+
+
+def _require_self(nhs, x, y, points, what):
+    if not _device.same_storage(x, y) or not _device.is_full(points, x.shape[1]):
+        raise ArgumentError(f"{what} needs x == y (the neighborhood search's own points)")
+    mode, _ = nhs._prepare(x, y, points)
+    if mode != PN_QUERIES_GRID:
+        raise ArgumentError(f"{what} needs the neighborhood search updated with x and all points active")
+
+
+class SPHPressureAcceleration(KernelFunctor):
+    """P is p / rho^2 per point."""
+
+    def __init__(self, dv, mass, P, smoothing_length, kernel="cubic_spline", accumulate=True):
+        self.dv, self.mass, self.P, self.h = dv, mass, P, float(smoothing_length)
+        self.kernel = _kernel_code(kernel)
+        self.accumulate = accumulate
+
+    def run_grid(self, nhs, x, y, points):
+        _require_self(nhs, x, y, points, "SPHPressureAcceleration")
+        td = _radius_torch_dtype(nhs)
+        n = x.shape[1]
+        _field(self.mass, n, td, "mass")
+        _field(self.P, n, td, "P")
+        _field(self.dv, n, td, "dv", nhs.ndims)
+        check(load().pn_sph_pressure_accel(nhs._handle, self.kernel, self.h, _vp(self.mass), _vp(self.P),
+                                           _vp(self.dv), 1 if self.accumulate else 0), "SPHPressureAcceleration")
+
+
+class WendlandPairEnergy(KernelFunctor):
+    def __init__(self, e, eps, cutoff, accumulate=True):
+        self.e, self.eps, self.cutoff, self.accumulate = e, float(eps), float(cutoff), accumulate
+
+    def run_grid(self, nhs, x, y, points):
+        mode, x = nhs._prepare(x, y, points)
+        _field(self.e, x.shape[1], _radius_torch_dtype(nhs), "e")
+        check(load().pn_pair_energy(nhs._handle, mode, self.eps, self.cutoff, _vp(self.e),
+                                    1 if self.accumulate else 0), "WendlandPairEnergy")
+
+
+def tait_eos(rho, rho0, c, gamma=7.0):
+    """p = B ((rho / rho0)^gamma - 1), B = rho0 c^2 / gamma, as a HIP kernel."""
+    import torch
+    p = torch.empty_like(rho)
+    code = 0 if rho.dtype == torch.float32 else 1
+    check(load().pn_tait_eos(_vp(rho), _vp(p), rho.numel(), code, float(rho0), float(c), float(gamma),
+                             ctypes.c_void_p(_device.current_stream_handle(rho.device))), "tait_eos")
+    return p
+
+
+def _vec_like(x):
+    """A zeroed D x N tensor with the coordinate layout (strides (1, D))."""
+    import torch
+    return torch.zeros((x.shape[1], x.shape[0]), dtype=x.dtype, device=x.device).t()
+
+
+def _grid_key(nhs):
+    return (nhs._y_key, id(nhs))
+
+
+def _make_functions():
+    import torch
+
+    class _Density(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, mass, nhs, h, kernel):
+            _require_self(nhs, x, x, None, "sph_density")
+            rho = torch.zeros_like(mass)
+            check(load().pn_sph_density(nhs._handle, PN_QUERIES_GRID, kernel, h, _vp(mass), _vp(rho), 0),
+                  "sph_density")
+            ctx.nhs, ctx.h, ctx.kernel, ctx.key = nhs, h, kernel, _grid_key(nhs)
+            ctx.save_for_backward(x, mass)
+            return rho
+
+        @staticmethod
+        def backward(ctx, rho_bar):
+            x, mass = ctx.saved_tensors
+            nhs = ctx.nhs
+            if _grid_key(nhs) != ctx.key:
+                raise PointNeighborsError("the neighborhood search changed between forward and backward")
+            rho_bar = rho_bar.contiguous()
+            xb = _vec_like(x).to(mass.dtype)
+            mb = torch.zeros_like(mass)
+            check(load().pn_sph_density_vjp(nhs._handle, ctx.kernel, ctx.h, _vp(mass), _vp(rho_bar), _vp(xb),
+                                            _vp(mb), 0), "sph_density backward")
+            return xb.to(x.dtype), mb, None, None, None
+
+    class _Pressure(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, mass, P, nhs, h, kernel):
+            _require_self(nhs, x, x, None, "sph_pressure_acceleration")
+            dv = _vec_like(x).to(mass.dtype)
+            check(load().pn_sph_pressure_accel(nhs._handle, kernel, h, _vp(mass), _vp(P), _vp(dv), 0),
+                  "sph_pressure_acceleration")
+            ctx.nhs, ctx.h, ctx.kernel, ctx.key = nhs, h, kernel, _grid_key(nhs)
+            ctx.save_for_backward(x, mass, P)
+            return dv
+
+        @staticmethod
+        def backward(ctx, dv_bar):
+            x, mass, P = ctx.saved_tensors
+            nhs = ctx.nhs
+            if _grid_key(nhs) != ctx.key:
+                raise PointNeighborsError("the neighborhood search changed between forward and backward")
+            ab = dv_bar.t().contiguous().t()
+            xb = _vec_like(x).to(mass.dtype)
+            Pb = torch.zeros_like(P)
+            mb = torch.zeros_like(mass)
+            check(load().pn_sph_pressure_accel_vjp(nhs._handle, ctx.kernel, ctx.h, _vp(mass), _vp(P), _vp(ab),
+                                                   _vp(xb), _vp(Pb), _vp(mb), 0),
+                  "sph_pressure_acceleration backward")
+            return xb.to(x.dtype), mb, Pb, None, None, None
+
+    class _Energy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, nhs, eps, cutoff):
+            _require_self(nhs, x, x, None, "pair_energy")
+            td = _radius_torch_dtype(nhs)
+            e = torch.zeros(x.shape[1], dtype=td, device=x.device)
+            check(load().pn_pair_energy(nhs._handle, PN_QUERIES_GRID, eps, cutoff, _vp(e), 0), "pair_energy")
+            ctx.nhs, ctx.eps, ctx.cutoff, ctx.key = nhs, eps, cutoff, _grid_key(nhs)
+            ctx.save_for_backward(x)
+            return e
+
+        @staticmethod
+        def backward(ctx, e_bar):
+            (x,) = ctx.saved_tensors
+            nhs = ctx.nhs
+            if _grid_key(nhs) != ctx.key:
+                raise PointNeighborsError("the neighborhood search changed between forward and backward")
+            e_bar = e_bar.contiguous()
+            xb = _vec_like(x).to(e_bar.dtype)
+            check(load().pn_pair_energy_vjp(nhs._handle, ctx.eps, ctx.cutoff, _vp(e_bar), _vp(xb), 0),
+                  "pair_energy backward")
+            return xb.to(x.dtype), None, None, None
+
+    return _Density, _Pressure, _Energy
+
+
+_FUNCS = None
+
+
+def _funcs():
+    global _FUNCS
+    if _FUNCS is None:
+        _FUNCS = _make_functions()
+    return _FUNCS
+
+
+def sph_density(x, mass, nhs, smoothing_length, kernel="cubic_spline"):
+    return _funcs()[0].apply(x, mass, nhs, float(smoothing_length), _kernel_code(kernel))
+
+
+def sph_pressure_acceleration(x, mass, P, nhs, smoothing_length, kernel="cubic_spline"):
+    return _funcs()[1].apply(x, mass, P, nhs, float(smoothing_length), _kernel_code(kernel))
+
+
+def pair_energy(x, nhs, eps, cutoff):
+    return _funcs()[2].apply(x, nhs, float(eps), float(cutoff))

@@ -172,13 +172,15 @@ def test_library_argument_errors_without_gpu():
 
 
 def test_traversal_lds_padding_fits():
-    """k_traverse pads each candidate chunk to S * L entries, L = ceil(mc / S) | 1; its
-    LDS arrays hold CH + 128 entries.  Check every (queries per cell, chunk size)."""
-    for ch in (512, 256):
-        worst = 0
+    """k_traverse holds CAP = CH + 64 staged candidates; a chunk takes at most
+    cmax = CAP - 2S + 1 of them and is padded to S * L, L = ceil(mc / S) | 1.  Check every
+    (queries per cell, chunk size) fits."""
+    for ch in (512, 256, 128):
+        cap = ch + 64
         for nq in range(1, 65):
             s = 64 // nq
-            for mc in range(1, ch + 1):
+            cmax = cap - 2 * s + 1
+            assert cmax >= 1
+            for mc in range(1, cmax + 1):
                 L = (mc + s - 1) // s | 1
-                worst = max(worst, s * L)
-        assert worst <= ch + 128
+                assert s * L <= cap

@@ -1,0 +1,122 @@
+"""Pins the oracle's interaction physics (parity unpinned against the reference: the
+physics lives in TrixiParticles.jl, absent here).  Checks that a dropped term, a wrong
+sign, normalization or index fails: kernel normalization by quadrature, closed-form
+single-pair values, Newton's third law, translation invariance, and dual-number
+derivatives against central finite differences.  CPU only."""
+
+import math
+
+import numpy as np
+import pytest
+
+from pointneighbors import synthetic
+
+
+def neighbor_lists(orc, pts, r, cap=1000):
+    lo, hi = pts.min(axis=0) - r, pts.max(axis=0) + r
+    pmin, _, ncells, _ = orc.fullgrid_geometry(lo, hi, r)
+    g = orc.Grid(pts.shape[1], pmin, ncells, r, max_points_per_cell=cap)
+    assert g.build(pts)[0] == orc.OK
+    st, _, lists = g.precomputed(pts, pts, 100000)
+    assert st == orc.OK
+    return lists
+
+
+@pytest.mark.parametrize("kernel", ["cubic_spline", "wendland_c2"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_kernel_normalization(orc, kernel, nd):
+    """sum_j dV W(|x_c - x_j|) over a fine lattice approximates the integral of W = 1."""
+    h = 1.0
+    n = 49 if nd == 2 else 33
+    dx = 4.4 * h / (n - 1)
+    axes = [np.arange(n) * dx - 2.2 * h for _ in range(nd)]
+    pts = np.stack([g.reshape(-1) for g in np.meshgrid(*axes, indexing="ij")], axis=1)
+    c = int(np.argmin(np.linalg.norm(pts, axis=1)))
+    # only the centre row: every lattice point within the support
+    nb = np.nonzero(np.linalg.norm(pts - pts[c], axis=1) <= 2.0 * h)[0].astype(np.int32)
+    rowptr = np.zeros(len(pts) + 1, dtype=np.int64)
+    rowptr[c + 1:] = len(nb)
+    rho, _ = orc.sph_density(pts, (rowptr, nb), np.full(len(pts), dx**nd), h, kernel)
+    assert abs(rho[c] - 1.0) < 2e-3
+
+
+@pytest.mark.parametrize("kernel,w0,w1", [("cubic_spline", 1.0, 0.25), ("wendland_c2", 1.0, 3 / 16)])
+def test_single_pair_values(orc, kernel, w0, w1):
+    h = 0.7
+    sigma = (1 / (math.pi * h**3)) if kernel == "cubic_spline" else 21 / (16 * math.pi * h**3)
+    pts = np.array([[0.0, 0.0, 0.0], [h, 0.0, 0.0]])
+    lists = (np.array([0, 2, 4]), np.array([0, 1, 0, 1], dtype=np.int32))
+    rho, _ = orc.sph_density(pts, lists, np.array([1.0, 2.0]), h, kernel)
+    # rho_0 = m_0 W(0) + m_1 W(h); W(0) = sigma, W(h) = sigma / 4 (cubic) or 3 sigma / 16 (Wendland)
+    assert rho[0] == pytest.approx(sigma * w0 + 2.0 * sigma * w1, rel=1e-14)
+    assert rho[1] == pytest.approx(2.0 * sigma * w0 + sigma * w1, rel=1e-14)
+    far = np.array([[0.0, 0.0, 0.0], [2 * h, 0.0, 0.0]])
+    rho, _ = orc.sph_density(far, lists, np.array([1.0, 1.0]), h, kernel)
+    assert rho[0] == pytest.approx(sigma, rel=1e-14)
+
+
+def test_pair_energy_values(orc):
+    eps, rc = 1.7, 2.5
+    for r, want in [(0.5 * rc, eps * 0.5**4 * 3.0), (rc, 0.0), (0.2 * rc, eps * 0.8**4 * 1.8)]:
+        pts = np.array([[0.0, 0.0], [r, 0.0]])
+        lists = (np.array([0, 2, 4]), np.array([0, 1, 0, 1], dtype=np.int32))
+        e = orc.pair_energy(pts, lists, eps, rc)
+        assert e.sum() == pytest.approx(want, rel=1e-14, abs=1e-300)
+        assert e[0] == pytest.approx(e[1], rel=1e-15, abs=1e-300)
+
+
+def cloud(shape, seed, jitter=0.2):
+    return synthetic.jittered_lattice(shape, 1.0, jitter, dtype=np.float64, seed=seed)
+
+
+def test_pressure_newton_third_law(orc):
+    pts = cloud((9, 10, 8), 3)
+    h = 1.2
+    lists = neighbor_lists(orc, pts, 2 * h)
+    m = np.full(len(pts), 0.8)
+    rho, _ = orc.sph_density(pts, lists, m, h)
+    P = orc.tait_eos(rho, 1.0, 10.0) / rho**2
+    dv, scale = orc.sph_pressure_accel(pts, lists, m, P, h)
+    total = (m[:, None] * dv).sum(axis=0)
+    assert np.abs(total).max() < 1e-10 * np.abs(m[:, None] * dv).sum()
+    # two points: a_1 = -m (P_1 + P_2) W'(r) (x_1 - x_2) / r, opposite to a_2
+    two = np.array([[0.0, 0.0, 0.0], [0.9, 0.3, -0.2]])
+    lists2 = (np.array([0, 2, 4]), np.array([0, 1, 0, 1], dtype=np.int32))
+    dv2, _ = orc.sph_pressure_accel(two, lists2, np.array([1.0, 1.0]), np.array([0.5, 1.5]), h)
+    assert np.allclose(dv2[0], -dv2[1], rtol=1e-14)
+    r = np.linalg.norm(two[0] - two[1])
+    # cubic spline W'(r) for q = r/h < 1: sigma/h (-3q + 9/4 q^2); the pair repels (P > 0)
+    q = r / h
+    dw = 1 / (math.pi * h**3) / h * (-3 * q + 2.25 * q * q)
+    want = -(0.5 + 1.5) * dw * (two[0] - two[1]) / r
+    assert np.allclose(dv2[0], want, rtol=1e-13)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_dual_matches_central_differences(orc, which):
+    pts = cloud((7, 8, 6), 5)
+    kw = dict(kernel="wendland_c2", h=1.25, eps=1.3, rc=2.5, rho0=1.0, c=1.0, gamma=7.0)
+    lists = neighbor_lists(orc, pts, 2.5)
+    m = np.full(len(pts), 0.9)
+    step = 1e-6
+    idx = (synthetic.uniform(17 + which, 6) * len(pts)).astype(int)
+    for a, k in enumerate(idx):
+        dim = a % 3
+        _, d = orc.loss_dual(which, pts, lists, k, dim, mass=m, **kw)
+        xp, xm = pts.copy(), pts.copy()
+        xp[k, dim] += step
+        xm[k, dim] -= step
+        lp, _ = orc.loss_dual(which, xp, lists, k, dim, mass=m, **kw)
+        lm, _ = orc.loss_dual(which, xm, lists, k, dim, mass=m, **kw)
+        fd = (lp - lm) / (2 * step)
+        assert abs(fd - d) <= 1e-6 * max(abs(d), 1e-3), (which, k, dim, d, fd)
+
+
+def test_energy_gradient_translation_invariance(orc):
+    pts = cloud((5, 6, 5), 7)
+    lists = neighbor_lists(orc, pts, 2.5)
+    g = np.zeros(3)
+    for k in range(len(pts)):
+        for dim in range(3):
+            g[dim] += orc.loss_dual(0, pts, lists, k, dim, eps=1.0, rc=2.5)[1]
+    assert np.abs(g).max() < 1e-12 * len(pts)

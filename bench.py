@@ -49,10 +49,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["wcsph", "count", "batch"], default="wcsph")
+    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr"], default="wcsph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the traversal launch and build strategy on this workload first")
+    ap.add_argument("--tuning-cache", default=None, help="JSON cache for --autotune results")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, copy 256 MiB device-to-device (a known-size kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -67,6 +73,12 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload == "batch":
         print(json.dumps(bench_batch(args, dev)))
+        return
+    if args.workload == "large":
+        print(json.dumps(bench_large(args, dev)))
+        return
+    if args.workload == "csr":
+        print(json.dumps(bench_csr(args, dev)))
         return
     pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
     n = pts.shape[0]
@@ -116,6 +128,11 @@ def main():
             ev[3].record(stream)
 
     pn.initialize(nhs, x, x)
+    tuned = None
+    if args.autotune:
+        from pointneighbors import autotune
+        key = autotune.signature(nhs, x, args.workload)
+        tuned, _ = autotune.autotune(nhs, step, reps=5, warmup=2, cache=args.tuning_cache, key=key)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -167,6 +184,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(pts, lo, hi, r, h, args.workload, args.cpu_seconds)
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        extra["copy_probe_bytes"] = src.numel() * 4
 
     line = {
         "metric": "neighbour pairs/s at 1M 3D Float32 points (whole step) + traversal % of HBM peak",
@@ -196,6 +219,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "algorithmic_bytes_per_launch": algo},
         "cpu_baseline": cpu,
+        "tuning": tuned,
     }
     print(json.dumps(line))
 
@@ -306,6 +330,121 @@ def bench_batch(args, dev):
         "per_cloud_ms_per_step": per_cloud_ms, "batched_eager_ms_per_step": eager_ms,
         "batched_graph_ms_per_step": graph_ms, "pairs_per_step": pairs,
         "pairs_per_s": pairs / (graph_ms * 1e-3),
+    }
+
+
+def _timed(fn, k, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(k):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / k
+
+
+def bench_large(args, dev):
+    """256^3 = 16,777,216 jittered lattice points (spacing 1, jitter 0.1), Float32,
+    h = 1.2, r = 2h: WCSPH forward (update! + density + EOS + pressure acceleration)
+    against forward + adjoint of L = sum |a_i|^2 with respect to the positions (reverse
+    mode through the pressure and density adjoint kernels; no stored pair lists)."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import sph, synthetic
+    pts = synthetic.jittered_lattice((256, 256, 256), 1.0, 0.1, dtype=np.float32, seed=1)
+    n = len(pts)
+    h = np.float32(1.2)
+    r = np.float32(2) * h
+    lo, hi = pts.min(axis=0) - 3 * r, pts.max(axis=0) + 3 * r
+    cl = pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=r)
+    nhs = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
+    x = synthetic.to_device_coords(pts, dev)
+    del pts
+    mass = torch.full((n,), 1000.0, dtype=torch.float32, device=dev)
+    rho = torch.zeros(n, dtype=torch.float32, device=dev)
+    P = torch.zeros(n, dtype=torch.float32, device=dev)
+    dv = torch.zeros((n, 3), dtype=torch.float32, device=dev).t()
+    f_rho = sph.SPHDensity(rho, mass, float(h), accumulate=False)
+    f_dv = sph.SPHPressureAcceleration(dv, mass, P, float(h), accumulate=False)
+    B = RHO0 * SOUND**2 / GAMMA
+    pn.initialize(nhs, x, x)
+
+    def forward():
+        pn.update(nhs, x, x)
+        pn.foreach_point_neighbor(f_rho, x, x, nhs)
+        p = sph.tait_eos(rho, RHO0, SOUND, GAMMA)
+        torch.div(p, rho * rho, out=P)
+        pn.foreach_point_neighbor(f_dv, x, x, nhs)
+
+    xg = x.detach().requires_grad_(True)
+
+    def forward_adjoint():
+        pn.update(nhs, x, x)
+        rh = sph.sph_density(xg, mass, nhs, float(h))
+        pp = B * ((rh / RHO0) ** GAMMA - 1.0)
+        a = sph.sph_pressure_acceleration(xg, mass, pp / (rh * rh), nhs, float(h))
+        loss = (a * a).sum()
+        (g,) = torch.autograd.grad(loss, xg)
+        return g
+
+    k = max(2, min(args.steps, 10))
+    fwd_ms = _timed(forward, k)
+    adj_ms = _timed(forward_adjoint, k)
+    g = forward_adjoint()
+    torch.cuda.synchronize()
+    c = torch.zeros(n, dtype=torch.int32, device=dev)
+    pn.foreach_point_neighbor(pn.CountNeighbors(c), x, x, nhs)
+    pairs = int(c.sum().item())
+    return {
+        "metric": "forward + adjoint time / forward time (16.8M 3D Float32 WCSPH step)",
+        "value": adj_ms / fwd_ms, "unit": "ratio", "n_gpus": 1, "steps": k, "warmup": 2,
+        "ms_per_step": adj_ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded SplitMix64 jittered lattice)",
+        "config": {"workload": "wcsph-16m-3d-f32: 256^3 lattice, spacing 1, jitter 0.1, h = 1.2, r = 2h; "
+                               "loss = sum |a|^2, adjoint w.r.t. positions", "n_points": n},
+        "forward_ms": fwd_ms, "forward_adjoint_ms": adj_ms, "pairs_per_step": pairs,
+        "forward_pairs_per_s": pairs / (fwd_ms * 1e-3),
+        "grad_finite": bool(torch.isfinite(g).all().item()),
+    }
+
+
+def bench_csr(args, dev):
+    """PrecomputedNeighborhoodSearch on an undeformed 128^3 lattice (spacing 1, Float32),
+    r = 3: CSR build (count, scan, fill, sort rows) and a TLSPH-style density sum over
+    the stored lists."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import sph, synthetic
+    pts = synthetic.jittered_lattice((128, 128, 128), 1.0, 0.0, dtype=np.float32, seed=1)
+    n = len(pts)
+    r = np.float32(3.0)
+    lo, hi = pts.min(axis=0) - r, pts.max(axis=0) + r
+    cl = pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=r)
+    inner = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
+    pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=r, n_points=n, update_neighborhood_search=inner)
+    x = synthetic.to_device_coords(pts, dev)
+    k = max(2, min(args.steps, 10))
+    build_ms = _timed(lambda: pn.initialize(pre, x, x), k)
+    nnz = int(pre.neighbor_lists.col_idx.numel())
+    lens = pre.neighbor_lists.lengths()
+    mass = torch.ones(n, dtype=torch.float32, device=dev)
+    rho = torch.zeros(n, dtype=torch.float32, device=dev)
+    f = sph.SPHDensity(rho, mass, 1.5, "wendland_c2", accumulate=False)
+    frozen = pn.freeze_neighborhood_search(pre)
+    sum_ms = _timed(lambda: pn.foreach_point_neighbor(f, x, x, frozen), k)
+    return {
+        "metric": "CSR neighbour-list build time (128^3 lattice, r = 3)", "value": build_ms, "unit": "ms",
+        "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": build_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (undeformed lattice)",
+        "config": {"workload": "csr-128cube-3d-f32: 128^3 lattice, spacing 1, r = 3; build + list density",
+                   "n_points": n},
+        "nnz": nnz, "max_row": int(lens.max().item()), "interior_row": int(lens[64 + 128 * 64 + 128 * 128 * 64].item()),
+        "build_ms": build_ms, "list_density_ms": sum_ms, "list_pairs_per_s": nnz / (sum_ms * 1e-3),
     }
 
 

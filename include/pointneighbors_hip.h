@@ -226,6 +226,16 @@ int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value);
 int pn_displace_uniform(void* x, int64_t n, int dtype, double amplitude, uint64_t seed,
                         int64_t* step, void* stream);
 
+/* SPH density over precomputed lists (a TLSPH-style sum; mapreduce_neighbor_inner of
+ * PrecomputedNeighborhoodSearch, src/nhs_precomputed.jl:210-247: every stored neighbour,
+ * distance recomputed, no radius test).  x, y: D x N (original order); mass per y point;
+ * rho per x point; rows selected like pn_grid_update's points.  Enqueue only. */
+int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
+                        int stored_base, int ndims, int coord_dtype, int radius_dtype, int kernel,
+                        double h, const void* x, const void* y, const void* mass, int64_t sel_first,
+                        int64_t sel_count, const int32_t* sel_list, void* rho, int accumulate,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

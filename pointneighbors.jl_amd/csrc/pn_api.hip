@@ -12,9 +12,9 @@
 
 #include "../../include/pointneighbors_hip.h"
 #include "pn_build.h"
-#include "pn_lists.h"
 #include "pn_sph.h"
 #include "pn_traverse.h"
+#include "pn_lists.h"
 
 namespace {
 
@@ -141,7 +141,9 @@ struct pn_grid {
     Buf geom_table, cloud_of, cell_offs;
     std::vector<pn::GeomEntry> host_table;
     std::vector<int64_t> host_point_offsets;
-    int64_t persist_blocks = 4096;  // workgroups of the persistent traversal (multiple of 8)
+    int64_t persist_blocks = 1 << 20;  // workgroups of the persistent traversal (multiple of 8)
+    int chunk = 0;                  // LDS candidate chunk: -1 per functor, 0 small, 1 large
+    int canonical_order = 1;        // 1: in-cell order = ascending original index
     Buf csr_counts;
     Buf csr_status;
     Scanner csr_scan;
@@ -264,7 +266,7 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
             hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
                                s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_first, sel_list, sel_count,
                                s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
-                               s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap);
+                               s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order);
             PN_CHECK_LAUNCH();
         }
         return PN_OK;
@@ -506,9 +508,8 @@ const BinSet* query_set(pn_grid& g, int queries) {
 // pair queues use smaller chunks: LDS per wavefront bounds the waves per CU.
 template <class TC, class F>
 struct Chunk {
-    static constexpr int light = sizeof(TC) == 4 ? 512 : 256;
-    static constexpr int heavy = sizeof(TC) == 4 ? 256 : 128;
-    static constexpr int value = F::kQueue ? heavy : light;
+    static constexpr int large = sizeof(TC) == 4 ? 512 : 256;
+    static constexpr int small = sizeof(TC) == 4 ? 256 : 128;
 };
 
 int check_queries(pn_grid& g, int queries, const BinSet*& Q) {
@@ -547,8 +548,13 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
-    hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::value>), dim3(nblocks), dim3(pn::WAVE), 0, g.stream,
-                       g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+    const bool large = g.chunk < 0 ? !F::kQueue : g.chunk == 1;
+    if (large)
+        hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::large>), dim3(nblocks), dim3(pn::WAVE), 0,
+                           g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+    else
+        hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::small>), dim3(nblocks), dim3(pn::WAVE), 0,
+                           g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
     PN_CHECK_LAUNCH();
     return PN_OK;
 }
@@ -965,6 +971,16 @@ int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value) {
         grid->persist_blocks = value;
         return PN_OK;
     }
+    if (k == "chunk") {
+        if (value < -1 || value > 1) return fail(PN_ERR_ARGUMENT, "chunk must be -1, 0 or 1");
+        grid->chunk = (int)value;
+        return PN_OK;
+    }
+    if (k == "canonical_order") {
+        if (value != 0 && value != 1) return fail(PN_ERR_ARGUMENT, "canonical_order must be 0 or 1");
+        grid->canonical_order = (int)value;
+        return PN_OK;
+    }
     return fail(PN_ERR_ARGUMENT, "unknown tuning key " + k);
 }
 
@@ -985,6 +1001,40 @@ int pn_displace_uniform(void* x, int64_t n, int dtype, double amplitude, uint64_
     hipLaunchKernelGGL(pn::k_step_increment, dim3(1), dim3(1), 0, s, step);
     PN_CHECK_LAUNCH();
     return PN_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base, int ndims,
+                        int coord_dtype, int radius_dtype, int kernel, double h, const void* x, const void* y,
+                        const void* mass, int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* rho,
+                        int accumulate, void* stream) {
+    if (!row_ptr || !x || !y || !mass || !rho) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the SPH kernels are defined for 2D and 3D");
+    if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
+        return fail(PN_ERR_ARGUMENT, "unknown smoothing kernel");
+    if (!(h > 0.0)) return fail(PN_ERR_ARGUMENT, "smoothing length must be positive");
+    if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
+        return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
+    if (sel_count <= 0) return PN_OK;
+    const int nblocks = (int)round_up8(sel_count);
+    pn_grid tmp;  // dispatch only
+    tmp.nd = ndims;
+    tmp.tc = coord_dtype;
+    tmp.tr = radius_dtype;
+    return dispatch_phys(tmp, [&](auto dc, auto tcv, auto trv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        auto kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        hipLaunchKernelGGL((pn::k_list_density<D, TC, TR, pn::SmoothingKernel<TR, D>>), dim3(nblocks), dim3(pn::WAVE),
+                           0, (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
+                           (const TC*)x, (const TC*)y, (const TR*)mass, kern, (TR*)rho, accumulate, nblocks);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    });
 }
 
 }  // extern "C"

@@ -290,7 +290,9 @@ __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ key
 constexpr int CANON_RANK_LIMIT = 1024;
 
 // Final position of each point: its cell's start plus the number of points of the same
-// cell with a smaller original index.  Writes the permutation and the cell-sorted
+// cell with a smaller original index (canonical = 1), or its provisional atomic slot
+// (canonical = 0: the order of the reference's ParallelUpdate, nondeterministic inside a
+// cell; neighbour sets are the same).  Writes the permutation and the cell-sorted
 // coordinate planes (plane d at coords + d * cap).
 template <int D, class TC>
 __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
                                                const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ perm_tmp,
                                                const TC* __restrict__ y, int32_t* __restrict__ sorted_idx,
-                                               TC* __restrict__ coords, int64_t cap) {
+                                               TC* __restrict__ coords, int64_t cap, int canonical) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_sel) return;
     const int key = keys[a];
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
     const int32_t p = (int32_t)sel_point(a, sel_first, sel_list);
     const int s = start[key], e = start[key + 1];
     int pos;
-    if (e - s <= CANON_RANK_LIMIT) {
+    if (canonical && e - s <= CANON_RANK_LIMIT) {
         int rank = 0;
         for (int q = s; q < e; ++q) rank += perm_tmp[q] < p;
         pos = s + rank;

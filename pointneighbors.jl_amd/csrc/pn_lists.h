@@ -72,4 +72,39 @@ __global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* __restri
     if (threadIdx.x == 0) out[i] = accumulate ? out[i] + s : s;
 }
 
+// SPH density over stored lists (the TLSPH-style sum of a PrecomputedNeighborhoodSearch,
+// src/nhs_precomputed.jl:210-247: stored neighbours, distance recomputed, no radius
+// test).  One wavefront per selected row, lanes stride the row (coalesced column reads),
+// wave reduction in a fixed order.  x and y are D x N coordinate matrices (original
+// order), mass per y point, rho per x point.
+template <int D, class TC, class TR, class K>
+__global__ __launch_bounds__(WAVE) void k_list_density(const int32_t* __restrict__ row_ptr,
+                                                       const int32_t* __restrict__ col, int stored_base,
+                                                       int64_t sel_first, const int32_t* __restrict__ sel_list,
+                                                       int64_t n_sel, const TC* __restrict__ x,
+                                                       const TC* __restrict__ y, const TR* __restrict__ mass,
+                                                       K kern, TR* __restrict__ rho, int accumulate, int nblocks) {
+    const int64_t a = xcd_remap(blockIdx.x, nblocks);
+    if (a >= n_sel) return;
+    const int64_t i = sel_point(a, sel_first, sel_list);
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    TC xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)D * i + d];
+    TR s = (TR)0;
+    for (int k = b + (int)threadIdx.x; k < e; k += WAVE) {
+        const int64_t j = (int64_t)col[k] - stored_base;
+        TR d2 = (TR)0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const TR p = (TR)(xi[d] - y[(int64_t)D * j + d]);
+            d2 = d2 + p * p;
+        }
+        s += mass[j] * kern.w(kern.sqrt_(d2));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
+    if (threadIdx.x == 0) rho[i] = accumulate ? rho[i] + s : s;
+}
+
 }  // namespace pn

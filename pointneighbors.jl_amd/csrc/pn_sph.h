@@ -60,6 +60,8 @@ struct SmoothingKernel {
         return k;
     }
 
+    __device__ __forceinline__ TR sqrt_(TR x) const { return psqrt(x); }
+
     // W(r), W'(r) = dW/dr, W''(r); branch-free in q (both pieces evaluated, selected)
     __device__ __forceinline__ TR w(TR r) const {
         const TR q = r * inv_h;

@@ -110,7 +110,10 @@ class BatchedGridNeighborhoodSearch:
                                         1 if accumulate else 0), "count_neighbors")
 
     def set_tuning(self, key, value):
-        check(load().pn_set_tuning(self._handle, key.encode(), int(value)))
+        """Same knobs as GridNeighborhoodSearch.set_tuning."""
+        if self._handle is None:
+            raise PointNeighborsError("update the batch once before tuning it")
+        check(load().pn_set_tuning(self._handle, key.encode(), int(value)), "set_tuning")
 
 
 def displace_uniform(x, amplitude, seed, step):

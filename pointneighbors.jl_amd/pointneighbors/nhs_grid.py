@@ -119,6 +119,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self._y_full = True     # eachindex_y covered all columns
         self._refreshed = False
         self._n_y = 0
+        self._tuning = {}
 
     # -- reference interface -------------------------------------------------------
     def requires_update(self):
@@ -157,6 +158,16 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self._handle = h
         self._coord_dtype = coord_dtype
         self._cell_dtype = cell_dt
+        for k, v in self._tuning.items():
+            check(lib.pn_set_tuning(h, k.encode(), int(v)), "set_tuning")
+
+    def set_tuning(self, key, value):
+        """Autotuner knobs: "persist_blocks" (traversal workgroups, multiple of 8),
+        "chunk" (-1 per functor, 0 small, 1 large LDS chunk), "canonical_order" (1: in-cell
+        order by original index; 0: atomic order, nondeterministic inside a cell)."""
+        self._tuning[key] = int(value)
+        if self._handle is not None:
+            check(load().pn_set_tuning(self._handle, key.encode(), int(value)), "set_tuning")
 
     def _set_stream(self, t):
         load().pn_grid_set_stream(self._handle, ctypes.c_void_p(_device.current_stream_handle(t.device)))

@@ -83,6 +83,25 @@ class SPHDensity(KernelFunctor):
         check(load().pn_sph_density(nhs._handle, mode, self.kernel, self.h, _vp(self.mass), _vp(self.rho),
                                     1 if self.accumulate else 0), "SPHDensity")
 
+    def run_lists(self, nhs, x, y, points):
+        """Over the stored lists (no radius test), src/nhs_precomputed.jl:210-247."""
+        from .util import dtype_code
+        nl = nhs.neighbor_lists
+        x = _device.as_coordinates(x, nhs.ndims, "system coordinates")
+        y = _device.as_coordinates(y, nhs.ndims, "neighbor coordinates")
+        td = _radius_torch_dtype(nhs)
+        _field(self.rho, x.shape[1], td, "rho")
+        _field(self.mass, y.shape[1], td, "mass")
+        first, count, ptr, keep = _device.selection(points, x.shape[1], x.device)
+        check(load().pn_list_sph_density(_vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, nhs.ndims,
+                                         dtype_code(_device.np_dtype(x)), dtype_code(nhs.radius_dtype),
+                                         self.kernel, self.h, _vp(x), _vp(y), _vp(self.mass), first, count,
+                                         ctypes.c_void_p(ptr) if ptr else None, _vp(self.rho),
+                                         1 if self.accumulate else 0,
+                                         ctypes.c_void_p(_device.current_stream_handle(x.device))),
+              "SPHDensity")
+        del keep
+
 
 def _require_self(nhs, x, y, points, what):
     if not _device.same_storage(x, y) or not _device.is_full(points, x.shape[1]):

@@ -182,3 +182,24 @@ def test_fd_density_loss_large(orc):
     pn.update(nhs, x, x)
     assert rel_err(np.array(rE), np.array(fdE)).max() < 1e-4
     assert rel_err(np.array(rL), np.array(fdL)).max() < 1e-4
+
+
+def test_csr_lattice_and_list_density(orc):
+    """Undeformed 24^3 lattice, r = 3: interior rows hold 123 neighbours, corner rows 29
+    (integer points in a ball / closed octant of radius 3); the density over the stored
+    lists equals the grid traversal's (summation order differs)."""
+    pts = synthetic.jittered_lattice((24, 24, 24), 1.0, 0.0, dtype=np.float32)
+    r = np.float32(3.0)
+    nhs = make_nhs(pts, r, pts.min(axis=0) - r, pts.max(axis=0) + r)
+    p = pn.PrecomputedNeighborhoodSearch(3, search_radius=r, n_points=len(pts), update_neighborhood_search=nhs)
+    x = dev(pts)
+    pn.initialize(p, x, x)
+    lens = p.neighbor_lists.lengths().cpu().numpy()
+    assert lens.max() == 123 and lens.min() == 29
+    assert lens[12 + 24 * 12 + 24 * 24 * 12] == 123 and lens[0] == 29
+    m = torch.full((len(pts),), 0.5, dtype=torch.float32, device="cuda")
+    rl = torch.zeros(len(pts), dtype=torch.float32, device="cuda")
+    rg = torch.zeros(len(pts), dtype=torch.float32, device="cuda")
+    pn.foreach_point_neighbor(sph.SPHDensity(rl, m, 1.5, "wendland_c2"), x, x, pn.freeze_neighborhood_search(p))
+    pn.foreach_point_neighbor(sph.SPHDensity(rg, m, 1.5, "wendland_c2"), x, x, nhs)
+    assert torch.allclose(rl, rg, rtol=1e-5, atol=0)

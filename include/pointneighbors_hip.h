@@ -79,6 +79,20 @@ int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
                    const double* min_corner, const int64_t* n_cells, double search_radius,
                    int32_t max_points_per_cell, pn_grid_t* out);
 int pn_grid_destroy(pn_grid_t grid);
+
+/* GridNeighborhoodSearch with a PeriodicBox (src/nhs_grid.jl:102-125, PeriodicBox
+ * src/neighborhood_search.jl:129-141).  The caller sizes the periodic grid as the
+ * reference constructor does:
+ *   n_periodic  floor.(Int, (box size .+ 10eps()) / search_radius), each >= 3
+ *   cell_size   box size ./ n_periodic, values exact in the radius type
+ *   box_size    max_corner - min_corner, values exact in the radius type
+ * Cell coordinates are then wrapped to mod.(cell .- 2, n_periodic) .+ 2
+ * (periodic_cell_index, src/nhs_grid.jl:609-620), neighbour cells wrap, and a candidate
+ * with d2 > r^2 is tested again at its minimum image (compute_periodic_distance,
+ * src/neighborhood_search.jl:428-436).  The cell list needs n_periodic + 2 cells per
+ * dimension.  Invalidates earlier builds; not available for batched clouds. */
+int pn_grid_set_periodic(pn_grid_t grid, const double* cell_size, const int64_t* n_periodic,
+                         const double* box_size);
 /* stream: a hipStream_t (NULL = default stream). */
 int pn_grid_set_stream(pn_grid_t grid, void* stream);
 
@@ -216,8 +230,12 @@ int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype
                     const double* min_corners, const int64_t* n_cells, double search_radius,
                     int32_t max_points_per_cell, const int64_t* point_offsets, pn_grid_t* out);
 
-/* Tuning knobs (the autotuner's choices).  No reference counterpart.
- *   "persist_blocks"  workgroups of the persistent traversal grid (multiple of 8) */
+/* Tuning knobs (the autotuner's choices).  No reference counterpart.  None changes a
+ * neighbour set; "algorithm" and "chunk" change the order of floating-point sums.
+ *   "algorithm"        0: candidate-major cells (default), 1: per-query candidate slices
+ *   "persist_blocks"   workgroups of the persistent traversal grid (multiple of 8)
+ *   "chunk"            candidates per chunk: -1 per functor, 0 small, 1 large
+ *   "canonical_order"  1: in-cell order by original index; 0: atomic order */
 int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value);
 
 /* Adds U[-amplitude, amplitude] to each of the n values of x (a pure function of seed,

@@ -55,6 +55,14 @@ def lib():
         L.orc_traverse.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]
         L.orc_trivial_count.restype = ci
         L.orc_trivial_count.argtypes = [ci, ci, ci, vp, vp, i64, i64, dbl, vp, vp, vp]
+        L.orc_trivial_count_periodic.restype = ci
+        L.orc_trivial_count_periodic.argtypes = [ci, ci, ci, vp, vp, i64, i64, dbl, vp, vp, vp, vp]
+        L.orc_grid_set_periodic.restype = None
+        L.orc_grid_set_periodic.argtypes = [vp, vp, vp, vp]
+        L.orc_grid_cell_coords.restype = None
+        L.orc_grid_cell_coords.argtypes = [vp, vp, vp]
+        L.orc_periodic_sizing.restype = None
+        L.orc_periodic_sizing.argtypes = [ci, ci, vp, dbl, vp, vp]
         L.orc_precomputed_lists.restype = ci
         L.orc_precomputed_lists.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp]
         L.orc_threaded_count_f32.restype = ci
@@ -102,6 +110,37 @@ def fullgrid_geometry(user_min, user_max, radius, user_dtype=np.float64, r_dtype
     return omin, omax, (None if tm < 0 else on), corner_dtype
 
 
+def periodic_sizing(box_size, radius, r_dtype=np.float64):
+    """(n_cells, cell_size) of a GridNeighborhoodSearch with a PeriodicBox."""
+    nd = len(box_size)
+    bs = np.asarray(box_size, dtype=np.float64)
+    n = np.zeros(nd, dtype=np.int64)
+    cs = np.zeros(nd)
+    lib().orc_periodic_sizing(nd, code(r_dtype), _ptr(bs), float(radius), _ptr(n), _ptr(cs))
+    return n, cs
+
+
+def trivial_periodic(x_nd, y_nd, radius, box_size, coord_dtype=np.float64, radius_dtype=np.float64,
+                     lists=False):
+    """TrivialNeighborhoodSearch with a PeriodicBox: counts (and sorted lists)."""
+    x = np.ascontiguousarray(x_nd, dtype=coord_dtype)
+    y = np.ascontiguousarray(y_nd, dtype=coord_dtype)
+    nd = x.shape[1]
+    bs = np.asarray(box_size, dtype=np.float64)
+    counts = np.zeros(x.shape[0], dtype=np.int32)
+    L = lib()
+    L.orc_trivial_count_periodic(nd, code(coord_dtype), code(radius_dtype), _ptr(x), _ptr(y), x.shape[0],
+                                 y.shape[0], float(radius), _ptr(bs), _ptr(counts), None, None)
+    if not lists:
+        return counts
+    rowptr = np.zeros(x.shape[0] + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(counts)
+    cols = np.zeros(max(int(rowptr[-1]), 1), dtype=np.int32)
+    L.orc_trivial_count_periodic(nd, code(coord_dtype), code(radius_dtype), _ptr(x), _ptr(y), x.shape[0],
+                                 y.shape[0], float(radius), _ptr(bs), None, _ptr(rowptr), _ptr(cols))
+    return counts, [sorted(cols[rowptr[i]:rowptr[i + 1]].tolist()) for i in range(x.shape[0])]
+
+
 class Grid:
     """Oracle GridNeighborhoodSearch + FullGridCellList (DVoV storage)."""
 
@@ -123,6 +162,20 @@ class Grid:
         if getattr(self, "h", None):
             lib().orc_grid_destroy(self.h)
             self.h = None
+
+    def set_periodic(self, cell_size, n_periodic, box_size):
+        """Periodic box: per-dimension cell size, periodic cell counts and box size."""
+        cs = np.asarray(cell_size, dtype=np.float64)
+        n = np.asarray(n_periodic, dtype=np.int64)
+        bs = np.asarray(box_size, dtype=np.float64)
+        lib().orc_grid_set_periodic(self.h, _ptr(cs), _ptr(n), _ptr(bs))
+
+    def cell_coords(self, x):
+        """1-based cell coordinates of one point (periodic wrap included)."""
+        xv = np.ascontiguousarray(x, dtype=self.tc)
+        out = np.zeros(3, dtype=np.int64)
+        lib().orc_grid_cell_coords(self.h, _ptr(xv), _ptr(out))
+        return tuple(int(v) for v in out[: self.nd])
 
     def _pts(self, a):
         a = np.ascontiguousarray(a, dtype=self.tc)

@@ -62,7 +62,11 @@ struct Geom {
     int ndims;
     int64_t n[3];
     double minc[3];   // padded min corner, exact in the cell type
-    double cs;        // cell size = search radius, exact in the cell type
+    double cs;        // search radius, exact in the cell type
+    double csd[3];    // cell size per dimension (the search radius unless periodic)
+    bool periodic = false;
+    int64_t per[3];   // periodic cells per dimension (src/nhs_grid.jl:116-117)
+    double box[3];    // periodic box size, exact in T_r
     int64_t ncells() const {
         int64_t c = 1;
         for (int d = 0; d < ndims; ++d) c *= n[d];
@@ -70,14 +74,45 @@ struct Geom {
     }
 };
 
+// Julia `mod(a, n)` for Int64 with n > 0: floored, result in [0, n).
+inline int64_t floored_mod(int64_t a, int64_t n) {
+    int64_t r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+// src/nhs_grid.jl:609-620 periodic_cell_index: mod.(cell .- 2, n_cells) .+ 2 (Int64 wrap)
+inline int64_t periodic_wrap(int64_t c, int64_t n) { return floored_mod(wrap_add(c, -2), n) + 2; }
+
 // src/cell_lists/full_grid.jl:84-94: floor_to_int.((x .- min_corner) ./ cell_size) .+ 1
-// The subtraction and division run in TCELL = promote(T_coords, T_min_corner).
+// The subtraction and division run in TCELL = promote(T_coords, T_min_corner).  With a
+// periodic box the result is wrapped (cell_coords, src/nhs_grid.jl:622-628).
 template <class TC, class TCELL>
 void cell_coords(const Geom& g, const TC* xp, int64_t cell[3]) {
     for (int d = 0; d < g.ndims; ++d) {
-        TCELL q = ((TCELL)xp[d] - (TCELL)g.minc[d]) / (TCELL)g.cs;
+        TCELL q = ((TCELL)xp[d] - (TCELL)g.minc[d]) / (TCELL)g.csd[d];
         cell[d] = wrap_add(floor_to_int<TCELL>(q), 1);
+        if (g.periodic) cell[d] = periodic_wrap(cell[d], g.per[d]);
     }
+}
+
+// compute_periodic_distance, src/neighborhood_search.jl:428-436: if d2 > r^2, take the
+// minimum image pos_diff -= size .* round.(pos_diff ./ size) (round half to even) and
+// recompute d2 = dot(pos_diff, pos_diff).
+template <class TR>
+inline TR periodic_distance(const Geom& g, TR pd[3], TR d2, TR r2) {
+    if (!g.periodic || !(d2 > r2)) return d2;
+    for (int d = 0; d < g.ndims; ++d) {
+        const TR L = (TR)g.box[d];
+        const TR k = std::nearbyint(pd[d] / L);
+        const TR s = L * k;
+        pd[d] = pd[d] - s;
+    }
+    TR e = pd[0] * pd[0];
+    for (int d = 1; d < g.ndims; ++d) {
+        TR p = pd[d] * pd[d];
+        e = e + p;
+    }
+    return e;
 }
 
 // src/cell_lists/full_grid.jl:187-213: every coordinate in 2:(n_d - 1)
@@ -174,12 +209,15 @@ int traverse_point(const Geom& g, const CellStore& cs, const TC* y, const TC* xi
         for (int64_t c2 = lo[1]; c2 <= hi[1]; ++c2)
             for (int64_t c1 = lo[0]; c1 <= hi[0]; ++c1) {
                 int64_t nc[3] = {c1, c2, c3};
+                if (g.periodic)  // points_in_cell wraps the neighbour cell, src/nhs_grid.jl:593-597
+                    for (int d = 0; d < D; ++d) nc[d] = periodic_wrap(nc[d], g.per[d]);
                 int64_t c = linear0(g, nc);
                 const int32_t* col = cs.backend.data() + c * cs.cap;
                 for (int32_t k = 0; k < cs.lengths[c]; ++k) {
                     int32_t j = col[k];
                     TR pd[3] = {0, 0, 0};
                     TR d2 = dist2<TC, TR>(D, xi, y + D * (int64_t)j, pd);
+                    d2 = periodic_distance<TR>(g, pd, d2, r2);
                     if (d2 <= r2) f(j, pd, d2);
                 }
             }
@@ -192,6 +230,9 @@ Geom make_geom(int ndims, const double* minc, double cs, const int64_t* n) {
     for (int d = 0; d < 3; ++d) {
         g.n[d] = d < ndims ? n[d] : 1;
         g.minc[d] = d < ndims ? minc[d] : 0.0;
+        g.csd[d] = cs;
+        g.per[d] = 1;
+        g.box[d] = 0.0;
     }
     g.cs = cs;
     return g;
@@ -345,6 +386,40 @@ void* orc_grid_create(int ndims, int tc, int tr, int tcell, const double* minc,
 
 void orc_grid_destroy(void* h) { delete (OracleGrid*)h; }
 
+// GridNeighborhoodSearch with a PeriodicBox (src/nhs_grid.jl:102-125): per-dimension
+// cell size (size ./ n_cells, exact in T_r), periodic cell counts and the box size.
+void orc_grid_set_periodic(void* h, const double* cell_size, const int64_t* n_periodic,
+                           const double* box_size) {
+    OracleGrid& og = *(OracleGrid*)h;
+    og.g.periodic = true;
+    for (int d = 0; d < og.g.ndims; ++d) {
+        og.g.csd[d] = cell_size[d];
+        og.g.per[d] = n_periodic[d];
+        og.g.box[d] = box_size[d];
+    }
+}
+
+// cell_coords of one point with the grid's geometry (periodic wrap included).
+void orc_grid_cell_coords(void* h, const void* x, int64_t* out) {
+    OracleGrid& og = *(OracleGrid*)h;
+    if (og.tc == 0 && og.tcell == 0) cell_coords<float, float>(og.g, (const float*)x, out);
+    else if (og.tc == 0) cell_coords<float, double>(og.g, (const float*)x, out);
+    else cell_coords<double, double>(og.g, (const double*)x, out);
+}
+
+// Periodic grid sizing, src/nhs_grid.jl:109-117:
+//   n_cells = floor.(Int, (size .+ 10eps()) / search_radius)   (Float64: 10eps() is Float64)
+//   cell_size = size ./ n_cells                                  (in T_r)
+void orc_periodic_sizing(int ndims, int r_dtype, const double* box_size, double radius,
+                         int64_t* out_n, double* out_cs) {
+    for (int d = 0; d < ndims; ++d) {
+        const double q = (box_size[d] + 10.0 * 2.220446049250313e-16) / radius;
+        out_n[d] = (int64_t)std::floor(q);
+        if (r_dtype == 0) out_cs[d] = (double)((float)box_size[d] / (float)out_n[d]);
+        else out_cs[d] = box_size[d] / (double)out_n[d];
+    }
+}
+
 #define ORC_DISPATCH(og, CALL)                                                        \
     do {                                                                              \
         if (og.tc == 0 && og.tr == 0 && og.tcell == 0) return Impl<float, float, float>::CALL; \
@@ -409,6 +484,45 @@ int orc_trivial_count(int ndims, int tc, int tr, const void* xv, const void* yv,
             for (int64_t j = 0; j < ny; ++j) {
                 TR pd[3] = {0, 0, 0};
                 TR d2 = dist2<TC, TR>(ndims, x + ndims * i, y + ndims * j, pd);
+                if (d2 <= r2) {
+                    ++cnt;
+                    if (rowptr) cols[pos++] = (int32_t)j;
+                }
+            }
+            if (counts) counts[i] = cnt;
+        }
+    };
+    if (tc == 0 && tr == 0) run(0.0f, 0.0f);
+    else if (tc == 0) run(0.0f, 0.0);
+    else if (tr == 0) run(0.0, 0.0f);
+    else run(0.0, 0.0);
+    return OK;
+}
+
+// TrivialNeighborhoodSearch with a PeriodicBox (src/nhs_trivial.jl:55, generic loop
+// src/neighborhood_search.jl:390-421 with compute_periodic_distance :428-436).
+int orc_trivial_count_periodic(int ndims, int tc, int tr, const void* xv, const void* yv, int64_t nx,
+                               int64_t ny, double radius, const double* box_size, int32_t* counts,
+                               const int64_t* rowptr, int32_t* cols) {
+    const int64_t ones[3] = {1, 1, 1};
+    const double zeros[3] = {0, 0, 0};
+    Geom g = make_geom(ndims, zeros, radius, ones);
+    g.periodic = true;
+    for (int d = 0; d < ndims; ++d) g.box[d] = box_size[d];
+    auto run = [&](auto tcv, auto trv) {
+        using TC = decltype(tcv);
+        using TR = decltype(trv);
+        const TC* x = (const TC*)xv;
+        const TC* y = (const TC*)yv;
+        const TR r = (TR)radius;
+        const TR r2 = r * r;
+        for (int64_t i = 0; i < nx; ++i) {
+            int32_t cnt = 0;
+            int64_t pos = rowptr ? rowptr[i] : 0;
+            for (int64_t j = 0; j < ny; ++j) {
+                TR pd[3] = {0, 0, 0};
+                TR d2 = dist2<TC, TR>(ndims, x + ndims * i, y + ndims * j, pd);
+                d2 = periodic_distance<TR>(g, pd, d2, r2);
                 if (d2 <= r2) {
                     ++cnt;
                     if (rowptr) cols[pos++] = (int32_t)j;

@@ -14,6 +14,7 @@
 #include "pn_build.h"
 #include "pn_sph.h"
 #include "pn_traverse.h"
+#include "pn_cells.h"
 #include "pn_lists.h"
 
 namespace {
@@ -142,7 +143,8 @@ struct pn_grid {
     std::vector<pn::GeomEntry> host_table;
     std::vector<int64_t> host_point_offsets;
     int64_t persist_blocks = 1 << 20;  // workgroups of the persistent traversal (multiple of 8)
-    int chunk = 0;                  // LDS candidate chunk: -1 per functor, 0 small, 1 large
+    int chunk = 0;                  // candidate chunk: -1 per functor, 0 small, 1 large
+    int algorithm = 0;              // traversal: 0 candidate-major cells, 1 per-query slices
     int canonical_order = 1;        // 1: in-cell order = ascending original index
     Buf csr_counts;
     Buf csr_status;
@@ -300,15 +302,21 @@ int describe_point(pn_grid& g, const void* src, int64_t p) {
     for (int d = 0; d < g.nd; ++d) {
         double q;
         if (g.tcell == PN_FLOAT32)
-            q = (double)(((float)xs[d] - (float)minc[d]) / (float)g.geom.cs);
+            q = (double)(((float)xs[d] - (float)minc[d]) / (float)g.geom.csd[d]);
         else
-            q = (xs[d] - minc[d]) / g.geom.cs;
+            q = (xs[d] - minc[d]) / g.geom.csd[d];
         double r = std::floor(q);
         int64_t c;
         if (std::isnan(r) || r >= 9223372036854775808.0) c = INT64_MAX;
         else if (r <= -9223372036854775808.0) c = INT64_MIN;
         else c = (int64_t)r;
-        g.err_cell[d] = (int64_t)((uint64_t)c + 1u);
+        c = (int64_t)((uint64_t)c + 1u);
+        if (g.geom.periodic) {
+            int64_t m = (int64_t)((uint64_t)c - 2u) % g.geom.per[d];
+            if (m < 0) m += g.geom.per[d];
+            c = m + 2;
+        }
+        g.err_cell[d] = c;
     }
     return PN_OK;
 }
@@ -384,7 +392,38 @@ int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
     }
     g->geom.ncells = (int)C;
     g->geom.cs = search_radius;
+    for (int d = 0; d < 3; ++d) {
+        g->geom.csd[d] = search_radius;
+        g->geom.per[d] = 1;
+        g->geom.box[d] = 0.0;
+    }
+    g->geom.periodic = 0;
     *out = g;
+    return PN_OK;
+}
+
+int pn_grid_set_periodic(pn_grid_t grid, const double* cell_size, const int64_t* n_periodic,
+                         const double* box_size) {
+    if (!grid || !cell_size || !n_periodic || !box_size) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    pn_grid& g = *grid;
+    if (g.geom.table) return fail(PN_ERR_UNSUPPORTED, "periodic boxes are not supported for batched clouds");
+    for (int d = 0; d < g.nd; ++d) {
+        if (n_periodic[d] < 3)
+            return fail(PN_ERR_ARGUMENT,
+                        "the `GridNeighborhoodSearch` needs at least 3 cells in each dimension when used with "
+                        "periodicity. Please use no NHS for very small problems.");
+        if (n_periodic[d] > g.geom.n[d] - 2)
+            return fail(PN_ERR_ARGUMENT, "the cell list has fewer than n_periodic + 2 cells in a dimension");
+        if (!(cell_size[d] > 0.0) || !(box_size[d] > 0.0)) return fail(PN_ERR_ARGUMENT, "box and cell sizes must be positive");
+    }
+    for (int d = 0; d < g.nd; ++d) {
+        g.geom.csd[d] = cell_size[d];
+        g.geom.per[d] = (int)n_periodic[d];
+        g.geom.box[d] = box_size[d];
+    }
+    g.geom.periodic = 1;
+    g.grid.valid = false;
+    g.query.valid = false;
     return PN_OK;
 }
 
@@ -548,6 +587,16 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
+    if (g.algorithm == 0 || g.geom.periodic) {  // only the cell kernel handles periodic boxes
+        if (g.chunk != 0)
+            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, pn::CellGroups<TC, F>::large>), dim3(nblocks),
+                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        else
+            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, pn::CellGroups<TC, F>::small>), dim3(nblocks),
+                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    }
     const bool large = g.chunk < 0 ? !F::kQueue : g.chunk == 1;
     if (large)
         hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::large>), dim3(nblocks), dim3(pn::WAVE), 0,
@@ -974,6 +1023,11 @@ int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value) {
     if (k == "chunk") {
         if (value < -1 || value > 1) return fail(PN_ERR_ARGUMENT, "chunk must be -1, 0 or 1");
         grid->chunk = (int)value;
+        return PN_OK;
+    }
+    if (k == "algorithm") {
+        if (value != 0 && value != 1) return fail(PN_ERR_ARGUMENT, "algorithm must be 0 or 1");
+        grid->algorithm = (int)value;
         return PN_OK;
     }
     if (k == "canonical_order") {

@@ -32,7 +32,11 @@ struct Geom {
     int n[3];      // cells per dimension, 1 for unused dimensions
     int ncells;    // total cells (all clouds)
     double minc[3];
-    double cs;
+    double cs;      // search radius
+    double csd[3];  // cell size per dimension: the search radius, or box size / per with a periodic box
+    int periodic;   // a PeriodicBox is set (src/nhs_grid.jl:102-125)
+    int per[3];     // periodic cells per dimension
+    double box[3];  // periodic box size
     const GeomEntry* table;
     const int32_t* cloud_of_point;
     const int32_t* cell_offsets;  // n_clouds + 1
@@ -96,22 +100,47 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// floor_to_int (src/util.jl:19-34) of an already floored value: NaN and values >= 2^63
+// saturate to typemax(Int64), values <= -2^63 to typemin(Int64).
+template <class T>
+__device__ __forceinline__ int64_t floor_to_int64(T f) {
+    if (!(f < (T)9223372036854775808.0)) return INT64_MAX;  // NaN too
+    if (f <= (T)-9223372036854775808.0) return INT64_MIN;
+    return (int64_t)f;
+}
+
 // Cell key of point p: (x - min) / cs in TCELL, floor, 1-based cell must lie in
 // 2:(n_d - 1) (src/cell_lists/full_grid.jl:84-94, :187-213).  With f = floor(q) the
 // condition reads 1 <= f <= n_d - 2; NaN and +-Inf fail both comparisons, matching the
 // saturating floor_to_int (src/util.jl:19-34) whose saturated values are out of range.
+// With a periodic box the 1-based cell floor_to_int(q) + 1 is wrapped to
+// mod(cell - 2, per) + 2 in Int64 arithmetic (src/nhs_grid.jl:609-628), so every
+// coordinate, NaN included, lands in 2:per+1; that must still lie in 2:(n_d - 1).
 template <int D, class TC, class TCELL>
-__device__ __forceinline__ int cell_key(const int (&n)[3], const double (&minc)[3], double cs,
-                                        const TC* __restrict__ xp, bool& ok) {
+__device__ __forceinline__ int cell_key(const int (&n)[3], const double (&minc)[3], const double (&cs)[3],
+                                        const Geom& g, const TC* __restrict__ xp, bool& ok) {
     int key = 0, stride = 1;
     ok = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        TCELL q = ((TCELL)xp[d] - (TCELL)minc[d]) / (TCELL)cs;
+        TCELL q = ((TCELL)xp[d] - (TCELL)minc[d]) / (TCELL)cs[d];
         TCELL f = floor(q);
-        bool in = (f >= (TCELL)1) && (f <= (TCELL)(n[d] - 2));
+        int c0;
+        bool in;
+        if (!g.periodic) {
+            in = (f >= (TCELL)1) && (f <= (TCELL)(n[d] - 2));
+            c0 = in ? (int)f : 0;
+        } else {
+            // (floor_to_int(q) + 1) - 2 with Int64 wrap, floored mod per
+            const int64_t a = (int64_t)((uint64_t)floor_to_int64(f) - 1ull);
+            int64_t r = a % (int64_t)g.per[d];
+            if (r < 0) r += g.per[d];
+            c0 = (int)r + 1;  // 0-based index of the 1-based cell r + 2
+            in = c0 <= n[d] - 2;
+            c0 = in ? c0 : 0;
+        }
         ok = ok && in;
-        key += (in ? (int)f : 0) * stride;
+        key += c0 * stride;
         stride *= n[d];
     }
     return key;
@@ -138,9 +167,9 @@ __global__ __launch_bounds__(256) void k_bin(Geom g, const TC* __restrict__ y, i
         int k;
         if (g.table) {
             const GeomEntry e = g.table[g.cloud_of_point[p]];
-            k = e.cell_off + cell_key<D, TC, TCELL>(e.n, e.minc, g.cs, y + (int64_t)D * p, ok);
+            k = e.cell_off + cell_key<D, TC, TCELL>(e.n, e.minc, g.csd, g, y + (int64_t)D * p, ok);
         } else {
-            k = cell_key<D, TC, TCELL>(g.n, g.minc, g.cs, y + (int64_t)D * p, ok);
+            k = cell_key<D, TC, TCELL>(g.n, g.minc, g.csd, g, y + (int64_t)D * p, ok);
         }
         if (ok) key = k;
         else atomicMin(&st->first_bad, (unsigned long long)a);

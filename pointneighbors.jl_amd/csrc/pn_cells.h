@@ -1,0 +1,400 @@
+// Candidate-major neighbour traversal (foreach_point_neighbor, hot loop
+// src/nhs_grid.jl:519-575) on gfx950: one wavefront per non-empty query cell.
+//
+// The 3^D cells around a query cell are 3^(D-1) x-rows of three adjacent cells, each a
+// contiguous range of the cell-sorted order.  The wavefront loads those candidates into
+// registers, 64 per group (lane = candidate), and broadcasts the cell's query points one at
+// a time through scalar registers.  Every group then tests its 64 candidates against the
+// query with the reference's predicate
+//     p = convert(T_r, x_i - y_j), d2 = ((p1*p1 + p2*p2) + p3*p3), accept iff d2 <= r*r
+// (src/nhs_grid.jl:547-555, built with -ffp-contract=off), nine VALU instructions per group
+// and no LDS traffic.
+//   * Counting: a group's accepted candidates are one ballot; its population count goes
+//     to the query's count in a scalar register.
+//   * Every other functor: the accepted (query, candidate) pairs are appended to a pair
+//     list in LDS in query-major order (per query in the reference's candidate order:
+//     stencil rows first-dimension fastest, then the in-cell order).  When the list is
+//     full, or the candidate chunk ends, the 64 lanes split it into 64 contiguous ranges
+//     of equal length and run the pair body.  A query whose pairs span several lanes is
+//     summed by the lane holding its first pair, adding the other lanes' partial sums in
+//     lane order, so sums are deterministic.
+#pragma once
+
+#include "pn_traverse.h"
+
+namespace pn {
+
+// Pair-list capacity (entries) of the candidate-major traversal.
+constexpr int PAIR_CAP = 1024;
+
+template <class T>
+__device__ __forceinline__ T lane_bcast(T v, int i) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+    } else {
+        const long long u = __builtin_bit_cast(long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffll), i);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), i);
+        return __builtin_bit_cast(T, (long long)(((unsigned long long)hi << 32) | lo));
+    }
+}
+
+// Candidate groups held in registers per chunk.
+template <class TC, class F>
+struct CellGroups {
+    static constexpr int large = F::kLight ? 8 : (sizeof(TC) == 4 ? 6 : 4);
+    static constexpr int small = F::kLight ? 8 : (sizeof(TC) == 4 ? 4 : 2);
+};
+
+template <int D, class TC, class TR, class F, int G>
+__global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, TR r2, F f, int nblocks) {
+    constexpr int R = Rows<D>::R;
+    constexpr int RP = 2 * R;  // stencil row pieces
+    constexpr int NF = F::kFields;
+    constexpr int CH = G * WAVE;
+    constexpr bool kLight = F::kLight;
+    using Acc = typename F::Acc;
+    using QS = typename F::QState;
+    constexpr int LC = kLight ? 1 : CH;
+    constexpr int LQ = kLight ? 1 : WAVE;
+    __shared__ Pack<TC, D> s_c[LC];
+    __shared__ int s_j[(!kLight && F::kIndex) ? CH : 1];
+    __shared__ TR s_f[NF > 0 ? NF : 1][(!kLight && NF > 0) ? CH : 1];
+    __shared__ uint16_t s_pairs[kLight ? 1 : PAIR_CAP];
+    __shared__ int s_qstart[LQ + 1];
+    __shared__ Pack<TC, D> s_qc[LQ];
+    __shared__ QS s_qst[LQ];
+    __shared__ Acc s_acc[LQ];
+    __shared__ Acc s_con[LQ];
+    __shared__ int s_written[F::kFill ? WAVE : 1];
+
+    const int lane = threadIdx.x;
+
+    // compute_periodic_distance (src/neighborhood_search.jl:428-436): if d2 > r^2 the
+    // minimum image pos_diff -= L .* round.(pos_diff ./ L) (round half to even) and d2 is
+    // recomputed.  round(pos_diff / L) is 0 unless |pos_diff| > L / 2, so lanes with every
+    // |pos_diff_d| <= 0.49 L_d keep pos_diff and d2 unchanged, as the reference would.
+    TR box[D], half[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        box[d] = (TR)g.box[d];
+        half[d] = (TR)0.49 * box[d];
+    }
+    auto periodic_fix = [&](TR (&pd)[D], TR d2) -> TR {
+        bool need = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) need = need || (fabs(pd[d]) > half[d]);
+        need = need && (d2 > r2);
+        if (__builtin_amdgcn_ballot_w64(need)) {
+            if (need) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const TR k = rint(pd[d] / box[d]);
+                    const TR s = box[d] * k;
+                    pd[d] = pd[d] - s;
+                }
+                d2 = pd[0] * pd[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) {
+                    const TR p2 = pd[d] * pd[d];
+                    d2 = d2 + p2;
+                }
+            }
+        }
+        return d2;
+    };
+
+    // Runs the pair body over pairs [0, P) of the list, which hold queries i0 .. i1-1
+    // (s_qstart[q - i0] = first pair of query q).  Lane t takes pairs
+    // [t*P/64, (t+1)*P/64).  A query inside one lane's range is added to s_acc by that
+    // lane; a query spanning lanes is summed by the lane holding its first pair, which
+    // adds the following lanes' partial sums (s_con) in lane order.
+    auto flush = [&](int i0, int i1, int P) {
+        const int nf = i1 - i0;
+        if (lane == 0) s_qstart[nf] = P;
+        s_con[lane] = f.init();
+        __syncthreads();
+        const int kb = (lane * P) >> 6, ke = ((lane + 1) * P) >> 6;
+        int own = -1;
+        Acc own_acc = f.init();
+        if (kb < ke) {
+            int lo = 0, hi = nf;  // largest fq with s_qstart[fq] <= kb
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qstart[mid] <= kb) lo = mid;
+                else hi = mid;
+            }
+            int fq = lo;
+            int next = s_qstart[fq + 1];
+            Pack<TC, D> qp = s_qc[i0 + fq];
+            QS qst = s_qst[i0 + fq];
+            Acc a = f.init();
+            auto finish = [&]() {
+                const bool starts = s_qstart[fq] >= kb;
+                if (starts && next <= ke) {
+                    Acc t = s_acc[i0 + fq];
+                    f.combine(t, a);
+                    s_acc[i0 + fq] = t;
+                } else if (starts) {
+                    own = fq;
+                    own_acc = a;
+                } else {
+                    s_con[lane] = a;
+                }
+            };
+            for (int k = kb; k < ke; ++k) {
+                if (k >= next) {
+                    finish();
+                    do {
+                        ++fq;
+                        next = s_qstart[fq + 1];
+                    } while (k >= next);
+                    qp = s_qc[i0 + fq];
+                    qst = s_qst[i0 + fq];
+                    a = f.init();
+                }
+                const int m = s_pairs[k];
+                const Pack<TC, D> pk = s_c[m];
+                TR pd[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) pd[d] = (TR)(qp.v[d] - pk.v[d]);
+                TR d2 = pd[0] * pd[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) {
+                    const TR p2 = pd[d] * pd[d];
+                    d2 = d2 + p2;
+                }
+                if (g.periodic) d2 = periodic_fix(pd, d2);
+                if constexpr (F::kFill) {
+                    const int slot = qst.rb + s_written[i0 + fq] + (k - s_qstart[fq]);
+                    if (slot < qst.re) f.col[slot] = s_j[m] + f.base;
+                } else {
+                    int j = 0;
+                    if constexpr (F::kIndex) j = s_j[m];
+                    TR fj[NF > 0 ? NF : 1];
+                    if constexpr (NF > 0) {
+#pragma unroll
+                        for (int fi = 0; fi < NF; ++fi) fj[fi] = s_f[fi][m];
+                    }
+                    f.template pair<TR, D>(a, qst, pd, d2, j, fj, nullptr);
+                }
+            }
+            finish();
+        }
+        __syncthreads();
+        if (own >= 0) {
+            const int e = s_qstart[own + 1];
+            for (int t = lane + 1; t < WAVE && ((t * P) >> 6) < e; ++t) f.combine(own_acc, s_con[t]);
+            Acc t = s_acc[i0 + own];
+            f.combine(t, own_acc);
+            s_acc[i0 + own] = t;
+        }
+        if constexpr (F::kFill) {
+            if (lane < nf) s_written[i0 + lane] += s_qstart[lane + 1] - s_qstart[lane];
+        }
+        __syncthreads();
+    };
+
+    const TC* __restrict__ gc = (const TC*)Gs.coords;
+    const TC* __restrict__ qcrd = (const TC*)Q.coords;
+    const TR* __restrict__ fs = (const TR*)f.fsorted;
+
+    // Persistent grid: XCD x (= blockIdx % 8) walks a contiguous range of the non-empty
+    // cells, so the cells that share stencil rows run on one XCD's L2.
+    const int n_work = Q.st->n_nonempty;
+    const int xcd = blockIdx.x & 7, per_xcd = nblocks >> 3;
+    const int wchunk = (n_work + 7) >> 3;
+    const int w_end = min(n_work, (xcd + 1) * wchunk);
+    for (int w = xcd * wchunk + (int)(blockIdx.x >> 3); w < w_end; w += per_xcd) {
+        const int qc = Q.nonempty[w];
+        const int qs = Q.start[qc], qe = Q.start[qc + 1];
+        int n0 = g.n[0], n1 = g.n[1];
+        if (g.table) {
+            const GeomEntry& e = g.table[cloud_of_cell(g, qc)];
+            n0 = e.n[0];
+            n1 = e.n[1];
+        }
+        // Stencil rows, each one or two pieces of the sorted order: piece u covers
+        // [rdel[u] + pre[u], rdel[u] + pre[u + 1]).  A row of 3 x-adjacent cells is one
+        // contiguous range; with a periodic box a row whose x-neighbour wraps is two
+        // (src/nhs_grid.jl:593-620), kept in the reference's cell order (x offset -1, 0, 1).
+        int rb = 0, rlen = 0;
+        if (lane < RP) {
+            const int t = lane >> 1, piece = lane & 1;
+            const int dy = D >= 2 ? (t % 3) - 1 : 0, dz = D >= 3 ? (t / 3) - 1 : 0;
+            if (!g.periodic) {
+                if (piece == 0) {
+                    const int base = qc - 1 + dy * n0 + dz * n0 * n1;
+                    rb = Gs.start[base];
+                    rlen = Gs.start[base + 3] - rb;
+                }
+            } else {
+                // 0-based periodic cells are 1..per; wrap(c) = ((c - 1) mod per) + 1
+                auto wrap = [](int c, int per) { return ((c - 1 + per) % per) + 1; };
+                const int cx = qc % n0, cy = (qc / n0) % n1, cz = qc / (n0 * n1);
+                const int yy = D >= 2 ? wrap(cy + dy, g.per[1]) : cy;
+                const int zz = D >= 3 ? wrap(cz + dz, g.per[2]) : cz;
+                const int row = yy * n0 + zz * n0 * n1;
+                const int xm = wrap(cx - 1, g.per[0]), xp = wrap(cx + 1, g.per[0]);
+                int lo, hi;  // cells [lo, hi) of this piece
+                if (xm == cx - 1 && xp == cx + 1) {
+                    lo = piece == 0 ? cx - 1 : 0;
+                    hi = piece == 0 ? cx + 2 : 0;
+                } else if (xm != cx - 1) {
+                    lo = piece == 0 ? xm : cx;
+                    hi = piece == 0 ? xm + 1 : cx + 2;
+                } else {
+                    lo = piece == 0 ? cx - 1 : xp;
+                    hi = piece == 0 ? cx + 1 : xp + 1;
+                }
+                if (hi > lo) {
+                    rb = Gs.start[row + lo];
+                    rlen = Gs.start[row + hi] - rb;
+                }
+            }
+        }
+        int rdel[RP], pre[RP + 1];  // sorted position of element m of piece u: m + rdel[u]
+        pre[0] = 0;
+#pragma unroll
+        for (int t = 0; t < RP; ++t) {
+            rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
+            pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
+        }
+        const int M = pre[RP];
+
+        for (int q0 = qs; q0 < qe; q0 += WAVE) {
+            const int nq = min(WAVE, qe - q0);
+            const bool qact = lane < nq;
+            TC xq[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) xq[d] = qact ? qcrd[(int64_t)d * Q.cap + q0 + lane] : (TC)0;
+            const int64_t orig_i = qact ? Q.sorted_idx[q0 + lane] : 0;
+            int mycount = 0;
+            if constexpr (!kLight) {
+                __syncthreads();
+                if (qact) {
+                    Pack<TC, D> pk;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pk.v[d] = xq[d];
+                    if constexpr (D == 3) pk.v[3] = (TC)0;
+                    s_qc[lane] = pk;
+                    s_qst[lane] = f.query(orig_i);
+                    s_acc[lane] = f.init();
+                    if constexpr (F::kFill) s_written[lane] = 0;
+                }
+            }
+
+            for (int m0 = 0; m0 < M; m0 += CH) {
+                const int mc = min(CH, M - m0);
+                const int ng = (mc + WAVE - 1) / WAVE;
+                TC cx[G][D];
+                if constexpr (!kLight) __syncthreads();  // the previous chunk's pairs are done
+                // Lanes past the chunk end hold +Inf (never accepted).
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi)
+                    if (gi == ng - 1) {
+#pragma unroll
+                        for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
+                    }
+                // Piece by piece: the groups a piece overlaps load its elements (m in the
+                // piece's range) from sorted position m + rdel[t].
+#pragma unroll
+                for (int t = 0; t < RP; ++t) {
+                    const int lo = max(pre[t], m0), hi = min(pre[t + 1], m0 + mc);
+                    if (lo < hi) {
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) {
+                            const int g_lo = m0 + gi * WAVE;
+                            if (g_lo < hi && g_lo + WAVE > lo) {
+                                const int m = g_lo + lane;
+                                if ((unsigned)(m - lo) < (unsigned)(hi - lo)) {
+                                    const unsigned src = (unsigned)(m + rdel[t]);
+                                    const int ml = gi * WAVE + lane;
+#pragma unroll
+                                    for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
+                                    if constexpr (!kLight) {
+                                        Pack<TC, D> pk;
+#pragma unroll
+                                        for (int d = 0; d < D; ++d) pk.v[d] = cx[gi][d];
+                                        if constexpr (D == 3) pk.v[3] = (TC)0;
+                                        s_c[ml] = pk;
+                                        if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
+                                        if constexpr (NF > 0) {
+#pragma unroll
+                                            for (int fi = 0; fi < NF; ++fi)
+                                                s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+
+                // d2 of broadcast query xs against group gi, in the reference's arithmetic
+                auto test = [&](const TC (&xs)[D], int gi) -> bool {
+                    TR pd[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pd[d] = (TR)(xs[d] - cx[gi][d]);
+                    TR d2 = pd[0] * pd[0];
+#pragma unroll
+                    for (int d = 1; d < D; ++d) {
+                        const TR p2 = pd[d] * pd[d];
+                        d2 = d2 + p2;
+                    }
+                    if (g.periodic) d2 = periodic_fix(pd, d2);
+                    return d2 <= r2;
+                };
+
+                if constexpr (kLight) {
+                    for (int i = 0; i < nq; ++i) {
+                        TC xs[D];
+#pragma unroll
+                        for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+                        int c = 0;
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi)
+                            if (gi < ng) c += __builtin_popcountll(__builtin_amdgcn_ballot_w64(test(xs, gi)));
+                        mycount += lane == i ? c : 0;
+                    }
+                } else {
+                    __syncthreads();  // staged candidates visible to every lane
+                    int P = 0, i0 = 0;
+                    for (int i = 0; i < nq; ++i) {
+                        if (P + ng * WAVE > PAIR_CAP) {
+                            flush(i0, i, P);
+                            i0 = i;
+                            P = 0;
+                        }
+                        if (lane == 0) s_qstart[i - i0] = P;
+                        TC xs[D];
+#pragma unroll
+                        for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) {
+                            if (gi < ng) {
+                                const bool acc = test(xs, gi);
+                                const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
+                                if (mask) {
+                                    const int below = __builtin_amdgcn_mbcnt_hi(
+                                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                                    if (acc) s_pairs[P + below] = (uint16_t)(gi * WAVE + lane);
+                                    P += __builtin_popcountll(mask);
+                                }
+                            }
+                        }
+                    }
+                    flush(i0, nq, P);
+                }
+            }
+            if constexpr (kLight) {
+                if (qact) f.write(orig_i, QS{}, mycount);
+            } else {
+                __syncthreads();
+                if (qact) f.write(orig_i, s_qst[lane], s_acc[lane]);
+            }
+        }
+    }
+}
+
+}  // namespace pn

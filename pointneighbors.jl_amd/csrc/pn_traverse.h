@@ -45,11 +45,13 @@ struct Vec {
 //   kQueue                   accepted candidates are queued per lane and the pair body
 //                            runs in lockstep over the queues (for bodies much costlier
 //                            than the distance test; ~1 in 6 candidates is accepted)
+//   kLight                   counting only (no pair body)
 struct FunctorBase {
     static constexpr bool kIndex = false;
     static constexpr int kFields = 0;
     static constexpr bool kFill = false;
     static constexpr bool kQueue = false;  // heavy pair body: run it on queued pairs only
+    static constexpr bool kLight = false;  // counting only: no pair body
     using QState = Empty;
     const void* fsorted = nullptr;
     int64_t fstride = 0;
@@ -58,6 +60,7 @@ struct FunctorBase {
 
 struct CountF : FunctorBase {
     using Acc = int;
+    static constexpr bool kLight = true;
     int32_t* out;
     int accumulate;
     __device__ Acc init() const { return 0; }
@@ -92,6 +95,10 @@ struct FillF : FunctorBase {
     int32_t* col;
     int base;
     int row_end;  // set per query by the traversal; guards the row against a stale count
+    struct QState {
+        int rb, re;  // row i of the CSR arrays: [rb, re)
+    };
+    __device__ QState query(int64_t i) const { return QState{row_ptr[i], row_ptr[i + 1]}; }
     __device__ Acc init() const { return 0; }
     __device__ int row_begin(int64_t i) const { return row_ptr[i]; }
     template <class TR, int D>

@@ -20,6 +20,7 @@ import json
 import os
 
 DEFAULT_SPACE = {
+    "algorithm": [0, 1],
     "persist_blocks": [2048, 8192, 32768, 131072],
     "chunk": [0, 1],
     "canonical_order": [1],

@@ -57,12 +57,38 @@ SUPPORTED_UPDATE_STRATEGIES = (ParallelUpdate, ParallelIncrementalUpdate, SemiPa
 
 
 class PeriodicBox:
-    """PeriodicBox(; min_corner, max_corner) (src/neighborhood_search.jl:129-141)."""
+    """PeriodicBox(; min_corner, max_corner) (src/neighborhood_search.jl:129-141): corners
+    and size = max_corner - min_corner in the corners' element type (Float32 or Float64)."""
 
     def __init__(self, *, min_corner, max_corner):
-        self.min_corner = np.asarray(min_corner, dtype=np.float64)
-        self.max_corner = np.asarray(max_corner, dtype=np.float64)
-        self.size = self.max_corner - self.min_corner
+        dt = promote(np.asarray(min_corner).dtype, np.asarray(max_corner).dtype)
+        dt = np.dtype(np.float32) if dt == np.float32 else np.dtype(np.float64)
+        self.min_corner = np.asarray(min_corner, dtype=dt)
+        self.max_corner = np.asarray(max_corner, dtype=dt)
+        self.size = (self.max_corner - self.min_corner).astype(dt)
+
+    def eltype(self):
+        return self.size.dtype
+
+    def __repr__(self):
+        return f"PeriodicBox(min_corner={self.min_corner.tolist()}, max_corner={self.max_corner.tolist()})"
+
+
+def periodic_grid_size(periodic_box, search_radius, radius_dtype):
+    """n_cells and cell_size of a GridNeighborhoodSearch with a PeriodicBox
+    (src/nhs_grid.jl:109-124).  n_cells = floor.(Int, (size .+ 10eps()) / search_radius)
+    evaluates in Float64 (10eps() is Float64); cell_size = size ./ n_cells in the radius
+    type."""
+    if np.dtype(radius_dtype) != periodic_box.eltype():
+        raise ArgumentError("the `search_radius` and the `PeriodicBox` must have the same element type")
+    size64 = periodic_box.size.astype(np.float64)
+    n = tuple(int(np.floor((v + 10.0 * FLOAT64_EPS) / float(search_radius))) for v in size64)
+    T = np.dtype(radius_dtype).type
+    cell_size = tuple(T(T(periodic_box.size[d]) / T(n[d])) for d in range(len(n)))
+    if any(v < 3 for v in n):
+        raise ArgumentError("the `GridNeighborhoodSearch` needs at least 3 cells in each dimension when "
+                            "used with periodicity. Please use no NHS for very small problems.")
+    return n, cell_size
 
 
 class _GridMeta(type):
@@ -100,8 +126,6 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         if r_dt.kind in "iub":
             raise ArgumentError("`search_radius` cannot be an integer type, since computed "
                                 "distances will be converted to this type")
-        if periodic_box is not None:
-            raise ArgumentError("PeriodicBox is not supported by the MI355X engine yet")
         self.ndims = ndims
         self.search_radius = search_radius
         self.radius_dtype = r_dt
@@ -109,8 +133,14 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self.periodic_box = periodic_box
         self.cell_list = cell_list
         self.update_strategy = update_strategy
-        self.cell_size = tuple(search_radius for _ in range(ndims))
-        self.n_cells = tuple(-1 for _ in range(ndims))
+        if float(search_radius) < FLOAT64_EPS or periodic_box is None:
+            # src/nhs_grid.jl:102-105: no periodicity
+            self.cell_size = tuple(search_radius for _ in range(ndims))
+            self.n_cells = tuple(-1 for _ in range(ndims))
+        else:
+            if len(periodic_box.size) != ndims:
+                raise ArgumentError(f"a {ndims}D PeriodicBox is required")
+            self.n_cells, self.cell_size = periodic_grid_size(periodic_box, search_radius, r_dt)
         self._handle = None
         self._coord_dtype = None
         self._built = False
@@ -158,13 +188,23 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self._handle = h
         self._coord_dtype = coord_dtype
         self._cell_dtype = cell_dt
+        if self._periodic():
+            pb = self.periodic_box
+            cs = (ctypes.c_double * nd)(*[float(v) for v in self.cell_size])
+            npc = (ctypes.c_int64 * nd)(*self.n_cells)
+            bs = (ctypes.c_double * nd)(*[float(v) for v in pb.size])
+            check(lib.pn_grid_set_periodic(h, cs, npc, bs), "GridNeighborhoodSearch")
         for k, v in self._tuning.items():
             check(lib.pn_set_tuning(h, k.encode(), int(v)), "set_tuning")
 
+    def _periodic(self):
+        return self.periodic_box is not None and self.n_cells[0] > 0
+
     def set_tuning(self, key, value):
-        """Autotuner knobs: "persist_blocks" (traversal workgroups, multiple of 8),
-        "chunk" (-1 per functor, 0 small, 1 large LDS chunk), "canonical_order" (1: in-cell
-        order by original index; 0: atomic order, nondeterministic inside a cell)."""
+        """Autotuner knobs: "algorithm" (0: candidate-major cells, 1: per-query candidate
+        slices), "persist_blocks" (traversal workgroups, multiple of 8), "chunk" (-1 per
+        functor, 0 small, 1 large candidate chunk), "canonical_order" (1: in-cell order by
+        original index; 0: atomic order, nondeterministic inside a cell)."""
         self._tuning[key] = int(value)
         if self._handle is not None:
             check(load().pn_set_tuning(self._handle, key.encode(), int(value)), "set_tuning")
@@ -178,12 +218,16 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         cl = self.cell_list
         ct = self._cell_dtype.type if self._handle is not None else promote(
             np.asarray(coords).dtype, cl.corner_dtype).type
-        cs = ct(self.search_radius)
         out = []
         for d in range(self.ndims):
+            cs = ct(self.cell_size[d])
             with np.errstate(all="ignore"):
                 q = (ct(coords[d]) - ct(cl.min_corner[d])) / cs
-            out.append(wrap_int64(floor_to_int(float(q)) + 1))
+            c = wrap_int64(floor_to_int(float(q)) + 1)
+            if self._periodic():
+                # periodic_cell_index, src/nhs_grid.jl:609-620
+                c = (wrap_int64(c - 2) % self.n_cells[d]) + 2
+            out.append(c)
         return tuple(out)
 
     # -- build -----------------------------------------------------------------------

@@ -74,8 +74,6 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
         if r_dt.kind in "iub":
             raise ArgumentError("`search_radius` cannot be an integer type, since computed "
                                 "distances will be converted to this type")
-        if periodic_box is not None:
-            raise ArgumentError("PeriodicBox is not supported by the MI355X engine yet")
         if update_neighborhood_search is not None and not isinstance(
                 update_neighborhood_search, GridNeighborhoodSearch):
             raise ArgumentError("update_neighborhood_search must be a GridNeighborhoodSearch "
@@ -103,6 +101,12 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
     def _auto_grid(self, x, y):
         import torch
         r = self.search_radius
+        if self.periodic_box is not None:
+            # the grid spans the periodic box; cells wrap, coordinates need not be inside
+            pb = self.periodic_box
+            cl = FullGridCellList(min_corner=pb.min_corner, max_corner=pb.max_corner, search_radius=r)
+            return GridNeighborhoodSearch(self.ndims, search_radius=r, n_points=self.n_points,
+                                          periodic_box=pb, cell_list=cl, update_strategy=self.update_strategy)
         both = torch.cat([x, y], dim=1) if x.shape[1] and y.shape[1] else (x if x.shape[1] else y)
         if both.shape[1] == 0:
             both = x

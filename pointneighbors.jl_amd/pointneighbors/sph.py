@@ -86,6 +86,9 @@ class SPHDensity(KernelFunctor):
     def run_lists(self, nhs, x, y, points):
         """Over the stored lists (no radius test), src/nhs_precomputed.jl:210-247."""
         from .util import dtype_code
+        if nhs.periodic_box is not None:
+            raise ArgumentError("SPHDensity over precomputed lists does not support a PeriodicBox; "
+                                "use the GridNeighborhoodSearch")
         nl = nhs.neighbor_lists
         x = _device.as_coordinates(x, nhs.ndims, "system coordinates")
         y = _device.as_coordinates(y, nhs.ndims, "neighbor coordinates")

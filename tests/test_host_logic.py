@@ -184,3 +184,60 @@ def test_traversal_lds_padding_fits():
             for mc in range(1, cmax + 1):
                 L = (mc + s - 1) // s | 1
                 assert s * L <= cap
+
+
+# ---- periodic boxes (src/nhs_grid.jl:102-125, src/neighborhood_search.jl:129-141) ----------
+
+def _periodic_nhs(lo, hi, r, dtype=np.float64):
+    lo = np.asarray(lo, dtype=dtype)
+    hi = np.asarray(hi, dtype=dtype)
+    pb = pn.PeriodicBox(min_corner=lo, max_corner=hi)
+    cl = pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=dtype(r))
+    return pn.GridNeighborhoodSearch(len(lo), search_radius=dtype(r), periodic_box=pb, cell_list=cl)
+
+
+def test_periodic_sizing_matches_oracle(orc, golden):
+    r = golden["periodic"]["search_radius"]
+    for case in golden["periodic"]["cases"]:
+        nhs = _periodic_nhs(case["min_corner"], case["max_corner"], r)
+        pb = nhs.periodic_box
+        n, cs = orc.periodic_sizing(pb.size, r)
+        assert list(nhs.n_cells) == n.tolist()
+        assert [float(v) for v in nhs.cell_size] == cs.tolist()
+    nhs = _periodic_nhs([0, 0, 0], [1.0, 1.3, 0.9], 0.29, dtype=np.float32)
+    n, cs = orc.periodic_sizing(nhs.periodic_box.size.astype(np.float64), float(np.float32(0.29)),
+                                r_dtype=np.float32)
+    assert list(nhs.n_cells) == n.tolist()
+    assert [float(v) for v in nhs.cell_size] == cs.tolist()
+    assert all(np.asarray(nhs.cell_size).dtype == np.float32 for _ in [0])
+
+
+def test_periodic_constructor_errors(golden):
+    E = golden["periodic"]["errors"]
+    pb = pn.PeriodicBox(min_corner=[0.0, 0.0], max_corner=[1.0, 1.0])
+    cl = pn.FullGridCellList(min_corner=[0.0, 0.0], max_corner=[1.0, 1.0], search_radius=np.float32(0.1))
+    with pytest.raises(pn.ArgumentError, match="must have the same element type"):
+        pn.GridNeighborhoodSearch(2, search_radius=np.float32(0.1), periodic_box=pb, cell_list=cl)
+    assert E["type_mismatch"].startswith("the `search_radius` and the `PeriodicBox`")
+    with pytest.raises(pn.ArgumentError, match="needs at least 3 cells in each dimension"):
+        _periodic_nhs([0.0, 0.0], [0.25, 1.0], 0.1)
+    # search_radius < eps(): no periodicity (src/nhs_grid.jl:102-105)
+    nhs = pn.GridNeighborhoodSearch(2, search_radius=0.0, periodic_box=pb,
+                                    cell_list=pn.FullGridCellList(min_corner=[0.0, 0.0], max_corner=[1.0, 1.0]))
+    assert nhs.n_cells == (-1, -1)
+
+
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+def test_periodic_host_cell_coords_match_oracle(orc, golden, T):
+    R = golden["periodic"]["rounding"]
+    nhs = _periodic_nhs(R["min_corner"], R["max_corner"], R["search_radius"], dtype=T)
+    cl = nhs.cell_list
+    g = orc.Grid(2, cl.min_corner.astype(np.float64), cl.n_cells, float(T(R["search_radius"])),
+                 coord_dtype=T, radius_dtype=T, cell_dtype=cl.corner_dtype)
+    g.set_periodic(np.asarray(nhs.cell_size, dtype=np.float64), nhs.n_cells,
+                   nhs.periodic_box.size.astype(np.float64))
+    vals = [np.nextafter(T(0), T(-1)), T(0), np.nextafter(T(0), T(1)), np.nextafter(T(1), T(0)), T(1),
+            np.nextafter(T(1), T(2)), T(-3.7), T(5.2), T(np.nan)]
+    for a in vals:
+        x = np.asarray([a, T(0.5)], dtype=T)
+        assert nhs.host_cell_coords(x) == g.cell_coords(x), a

@@ -254,6 +254,17 @@ int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
                         int64_t sel_count, const int32_t* sel_list, void* rho, int accumulate,
                         void* stream);
 
+/* pn_list_sph_density of a PrecomputedNeighborhoodSearch with a PeriodicBox: a stored pair
+ * with d2 > search_radius^2 uses its minimum image (compute_periodic_distance,
+ * src/neighborhood_search.jl:428-436, called at src/nhs_precomputed.jl:236-238).
+ * box_size: max_corner - min_corner, values exact in the radius type.  Enqueue only. */
+int pn_list_sph_density_periodic(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
+                                 int stored_base, int ndims, int coord_dtype, int radius_dtype,
+                                 int kernel, double h, const void* x, const void* y,
+                                 const void* mass, int64_t sel_first, int64_t sel_count,
+                                 const int32_t* sel_list, void* rho, int accumulate,
+                                 double search_radius, const double* box_size, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

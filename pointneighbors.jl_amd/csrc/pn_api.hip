@@ -1061,10 +1061,11 @@ int pn_displace_uniform(void* x, int64_t n, int dtype, double amplitude, uint64_
 
 extern "C" {
 
-int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base, int ndims,
-                        int coord_dtype, int radius_dtype, int kernel, double h, const void* x, const void* y,
-                        const void* mass, int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* rho,
-                        int accumulate, void* stream) {
+static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
+                            int ndims, int coord_dtype, int radius_dtype, int kernel, double h, const void* x,
+                            const void* y, const void* mass, int64_t sel_first, int64_t sel_count,
+                            const int32_t* sel_list, void* rho, int accumulate, void* stream, double search_radius,
+                            const double* box_size) {
     if (!row_ptr || !x || !y || !mass || !rho) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the SPH kernels are defined for 2D and 3D");
     if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
@@ -1072,6 +1073,9 @@ int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
     if (!(h > 0.0)) return fail(PN_ERR_ARGUMENT, "smoothing length must be positive");
     if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
         return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
+    if (box_size)
+        for (int d = 0; d < ndims; ++d)
+            if (!(box_size[d] > 0.0)) return fail(PN_ERR_ARGUMENT, "box sizes must be positive");
     if (sel_count <= 0) return PN_OK;
     const int nblocks = (int)round_up8(sel_count);
     pn_grid tmp;  // dispatch only
@@ -1083,12 +1087,35 @@ int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
         using TC = decltype(tcv);
         using TR = decltype(trv);
         auto kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
+        pn::ListPeriodic<TR> per{};
+        per.on = box_size ? 1 : 0;
+        const TR r = (TR)search_radius;
+        per.r2 = r * r;
+        for (int d = 0; d < 3; ++d) per.box[d] = (box_size && d < D) ? (TR)box_size[d] : (TR)1;
         hipLaunchKernelGGL((pn::k_list_density<D, TC, TR, pn::SmoothingKernel<TR, D>>), dim3(nblocks), dim3(pn::WAVE),
                            0, (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
-                           (const TC*)x, (const TC*)y, (const TR*)mass, kern, (TR*)rho, accumulate, nblocks);
+                           (const TC*)x, (const TC*)y, (const TR*)mass, kern, (TR*)rho, accumulate, nblocks, per);
         PN_CHECK_LAUNCH();
         return PN_OK;
     });
+}
+
+int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base, int ndims,
+                        int coord_dtype, int radius_dtype, int kernel, double h, const void* x, const void* y,
+                        const void* mass, int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* rho,
+                        int accumulate, void* stream) {
+    return list_sph_density(row_ptr, col_idx, n_rows, stored_base, ndims, coord_dtype, radius_dtype, kernel, h, x, y,
+                            mass, sel_first, sel_count, sel_list, rho, accumulate, stream, 0.0, nullptr);
+}
+
+int pn_list_sph_density_periodic(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
+                                 int ndims, int coord_dtype, int radius_dtype, int kernel, double h, const void* x,
+                                 const void* y, const void* mass, int64_t sel_first, int64_t sel_count,
+                                 const int32_t* sel_list, void* rho, int accumulate, double search_radius,
+                                 const double* box_size, void* stream) {
+    if (!box_size) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    return list_sph_density(row_ptr, col_idx, n_rows, stored_base, ndims, coord_dtype, radius_dtype, kernel, h, x, y,
+                            mass, sel_first, sel_count, sel_list, rho, accumulate, stream, search_radius, box_size);
 }
 
 }  // extern "C"

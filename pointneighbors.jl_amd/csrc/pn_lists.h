@@ -72,6 +72,15 @@ __global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* __restri
     if (threadIdx.x == 0) out[i] = accumulate ? out[i] + s : s;
 }
 
+// Periodic box of a list traversal: compute_periodic_distance
+// (src/neighborhood_search.jl:428-436) applies when on != 0.
+template <class TR>
+struct ListPeriodic {
+    int on;
+    TR r2;
+    TR box[3];
+};
+
 // SPH density over stored lists (the TLSPH-style sum of a PrecomputedNeighborhoodSearch,
 // src/nhs_precomputed.jl:210-247: stored neighbours, distance recomputed, no radius
 // test).  One wavefront per selected row, lanes stride the row (coalesced column reads),
@@ -83,7 +92,8 @@ __global__ __launch_bounds__(WAVE) void k_list_density(const int32_t* __restrict
                                                        int64_t sel_first, const int32_t* __restrict__ sel_list,
                                                        int64_t n_sel, const TC* __restrict__ x,
                                                        const TC* __restrict__ y, const TR* __restrict__ mass,
-                                                       K kern, TR* __restrict__ rho, int accumulate, int nblocks) {
+                                                       K kern, TR* __restrict__ rho, int accumulate, int nblocks,
+                                                       ListPeriodic<TR> per) {
     const int64_t a = xcd_remap(blockIdx.x, nblocks);
     if (a >= n_sel) return;
     const int64_t i = sel_point(a, sel_first, sel_list);
@@ -94,11 +104,28 @@ __global__ __launch_bounds__(WAVE) void k_list_density(const int32_t* __restrict
     TR s = (TR)0;
     for (int k = b + (int)threadIdx.x; k < e; k += WAVE) {
         const int64_t j = (int64_t)col[k] - stored_base;
-        TR d2 = (TR)0;
+        TR pd[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const TR p = (TR)(xi[d] - y[(int64_t)D * j + d]);
-            d2 = d2 + p * p;
+        for (int d = 0; d < D; ++d) pd[d] = (TR)(xi[d] - y[(int64_t)D * j + d]);
+        TR d2 = pd[0] * pd[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+            const TR p2 = pd[d] * pd[d];
+            d2 = d2 + p2;
+        }
+        if (per.on && d2 > per.r2) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const TR kk = rint(pd[d] / per.box[d]);
+                const TR sh = per.box[d] * kk;
+                pd[d] = pd[d] - sh;
+            }
+            d2 = pd[0] * pd[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const TR p2 = pd[d] * pd[d];
+                d2 = d2 + p2;
+            }
         }
         s += mass[j] * kern.w(kern.sqrt_(d2));
     }

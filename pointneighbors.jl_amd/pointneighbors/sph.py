@@ -86,9 +86,6 @@ class SPHDensity(KernelFunctor):
     def run_lists(self, nhs, x, y, points):
         """Over the stored lists (no radius test), src/nhs_precomputed.jl:210-247."""
         from .util import dtype_code
-        if nhs.periodic_box is not None:
-            raise ArgumentError("SPHDensity over precomputed lists does not support a PeriodicBox; "
-                                "use the GridNeighborhoodSearch")
         nl = nhs.neighbor_lists
         x = _device.as_coordinates(x, nhs.ndims, "system coordinates")
         y = _device.as_coordinates(y, nhs.ndims, "neighbor coordinates")
@@ -96,13 +93,16 @@ class SPHDensity(KernelFunctor):
         _field(self.rho, x.shape[1], td, "rho")
         _field(self.mass, y.shape[1], td, "mass")
         first, count, ptr, keep = _device.selection(points, x.shape[1], x.device)
-        check(load().pn_list_sph_density(_vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, nhs.ndims,
-                                         dtype_code(_device.np_dtype(x)), dtype_code(nhs.radius_dtype),
-                                         self.kernel, self.h, _vp(x), _vp(y), _vp(self.mass), first, count,
-                                         ctypes.c_void_p(ptr) if ptr else None, _vp(self.rho),
-                                         1 if self.accumulate else 0,
-                                         ctypes.c_void_p(_device.current_stream_handle(x.device))),
-              "SPHDensity")
+        args = (_vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, nhs.ndims, dtype_code(_device.np_dtype(x)),
+                dtype_code(nhs.radius_dtype), self.kernel, self.h, _vp(x), _vp(y), _vp(self.mass), first, count,
+                ctypes.c_void_p(ptr) if ptr else None, _vp(self.rho), 1 if self.accumulate else 0)
+        stream = ctypes.c_void_p(_device.current_stream_handle(x.device))
+        if nhs.periodic_box is not None:
+            pb = nhs.periodic_box
+            box = (ctypes.c_double * nhs.ndims)(*[float(v) for v in pb.size])
+            check(load().pn_list_sph_density_periodic(*args, float(nhs.search_radius), box, stream), "SPHDensity")
+        else:
+            check(load().pn_list_sph_density(*args, stream), "SPHDensity")
         del keep
 
 

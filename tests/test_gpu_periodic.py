@@ -125,3 +125,25 @@ def test_periodic_lattice_uniform_density(T):
     pn.foreach_point_neighbor(sph.SPHDensity(rho2, mass, float(h)), x, x, plain)
     centre = (m // 2 - 1) + m * (m // 2 - 1) + m * m * (m // 2 - 1)
     assert np.isclose(rho2.cpu().numpy()[centre], rho[0], rtol=tol, atol=0)
+
+
+def test_periodic_list_density_matches_grid():
+    """PrecomputedNeighborhoodSearch with a PeriodicBox: the list density (minimum image of
+    the stored pairs) equals the grid traversal's density."""
+    T = np.float64
+    rng = np.random.default_rng(5)
+    lo, hi = np.zeros(3), np.array([8.0, 9.0, 7.5])
+    pts = (lo + rng.uniform(-0.02, 1.02, size=(900, 3)) * (hi - lo)).astype(T)
+    h = 1.1
+    nhs = periodic_nhs(lo, hi, 2 * h, T)
+    x = dev(pts)
+    pn.initialize(nhs, x, x)
+    mass = torch.rand(len(pts), dtype=x.dtype, device=x.device) + 0.5
+    rho_g = torch.zeros(len(pts), dtype=x.dtype, device=x.device)
+    pn.foreach_point_neighbor(sph.SPHDensity(rho_g, mass, h), x, x, nhs)
+    p = pn.PrecomputedNeighborhoodSearch(3, search_radius=T(2 * h), n_points=len(pts),
+                                         periodic_box=nhs.periodic_box)
+    pn.initialize(p, x, x)
+    rho_l = torch.zeros_like(rho_g)
+    pn.foreach_point_neighbor(sph.SPHDensity(rho_l, mass, h), x, x, p)
+    assert torch.allclose(rho_l, rho_g, rtol=1e-12, atol=0)

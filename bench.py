@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune the traversal launch and build strategy on this workload first")
     ap.add_argument("--tuning-cache", default=None, help="JSON cache for --autotune results")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a tuning knob (algorithm, persist_blocks, chunk, canonical_order) "
+                         "before timing; repeatable")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, copy 256 MiB device-to-device (a known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -133,6 +136,12 @@ def main():
         from pointneighbors import autotune
         key = autotune.signature(nhs, x, args.workload)
         tuned, _ = autotune.autotune(nhs, step, reps=5, warmup=2, cache=args.tuning_cache, key=key)
+    if args.tune:
+        tuned = dict(tuned or {})
+        for kv in args.tune:
+            k, v = kv.split("=", 1)
+            nhs.set_tuning(k, int(v))
+            tuned[k] = int(v)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -588,12 +588,21 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
     if (g.algorithm == 0 || g.geom.periodic) {  // only the cell kernel handles periodic boxes
-        if (g.chunk != 0)
-            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, pn::CellGroups<TC, F>::large>), dim3(nblocks),
-                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
-        else
-            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, pn::CellGroups<TC, F>::small>), dim3(nblocks),
-                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        auto launch = [&](auto gc, auto perc) {
+            constexpr int GG = decltype(gc)::value;
+            constexpr bool PER = decltype(perc)::value;
+            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, GG, PER>), dim3(nblocks), dim3(pn::WAVE), 0, g.stream,
+                               g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        };
+        using Large = pn::IntC<pn::CellGroups<TC, F>::large>;
+        using Small = pn::IntC<pn::CellGroups<TC, F>::small>;
+        if (g.geom.periodic) {
+            if (g.chunk != 0) launch(Large{}, std::true_type{});
+            else launch(Small{}, std::true_type{});
+        } else {
+            if (g.chunk != 0) launch(Large{}, std::false_type{});
+            else launch(Small{}, std::false_type{});
+        }
         PN_CHECK_LAUNCH();
         return PN_OK;
     }

@@ -22,6 +22,8 @@
 
 #include "pn_traverse.h"
 
+#include <type_traits>
+
 namespace pn {
 
 // Pair-list capacity (entries) of the candidate-major traversal.
@@ -46,10 +48,25 @@ struct CellGroups {
     static constexpr int small = F::kLight ? 8 : (sizeof(TC) == 4 ? 4 : 2);
 };
 
-template <int D, class TC, class TR, class F, int G>
+template <int N>
+using IntC = std::integral_constant<int, N>;
+
+// Calls fn(IntC<n>{}) for a runtime n in 1..G, so loops over n groups unroll without guards.
+template <int G, class Fn>
+__device__ __forceinline__ void with_groups(int n, Fn&& fn) {
+    if constexpr (G >= 1) {
+        if (n == G) {
+            fn(IntC<G>{});
+            return;
+        }
+        with_groups<G - 1>(n, fn);
+    }
+}
+
+template <int D, class TC, class TR, class F, int G, bool PER>
 __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, TR r2, F f, int nblocks) {
     constexpr int R = Rows<D>::R;
-    constexpr int RP = 2 * R;  // stencil row pieces
+    constexpr int RP = PER ? 2 * R : R;  // stencil row pieces
     constexpr int NF = F::kFields;
     constexpr int CH = G * WAVE;
     constexpr bool kLight = F::kLight;
@@ -60,7 +77,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ Pack<TC, D> s_c[LC];
     __shared__ int s_j[(!kLight && F::kIndex) ? CH : 1];
     __shared__ TR s_f[NF > 0 ? NF : 1][(!kLight && NF > 0) ? CH : 1];
-    __shared__ uint16_t s_pairs[kLight ? 1 : PAIR_CAP];
+    __shared__ uint16_t s_pairs[kLight ? 1 : PAIR_CAP + WAVE];  // + one discard slot per lane
     __shared__ int s_qstart[LQ + 1];
     __shared__ Pack<TC, D> s_qc[LQ];
     __shared__ QS s_qst[LQ];
@@ -164,7 +181,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     const TR p2 = pd[d] * pd[d];
                     d2 = d2 + p2;
                 }
-                if (g.periodic) d2 = periodic_fix(pd, d2);
+                if constexpr (PER) d2 = periodic_fix(pd, d2);
                 if constexpr (F::kFill) {
                     const int slot = qst.rb + s_written[i0 + fq] + (k - s_qstart[fq]);
                     if (slot < qst.re) f.col[slot] = s_j[m] + f.base;
@@ -220,14 +237,12 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         // (src/nhs_grid.jl:593-620), kept in the reference's cell order (x offset -1, 0, 1).
         int rb = 0, rlen = 0;
         if (lane < RP) {
-            const int t = lane >> 1, piece = lane & 1;
+            const int t = PER ? lane >> 1 : lane, piece = PER ? lane & 1 : 0;
             const int dy = D >= 2 ? (t % 3) - 1 : 0, dz = D >= 3 ? (t / 3) - 1 : 0;
-            if (!g.periodic) {
-                if (piece == 0) {
-                    const int base = qc - 1 + dy * n0 + dz * n0 * n1;
-                    rb = Gs.start[base];
-                    rlen = Gs.start[base + 3] - rb;
-                }
+            if constexpr (!PER) {
+                const int base = qc - 1 + dy * n0 + dz * n0 * n1;
+                rb = Gs.start[base];
+                rlen = Gs.start[base + 3] - rb;
             } else {
                 // 0-based periodic cells are 1..per; wrap(c) = ((c - 1) mod per) + 1
                 auto wrap = [](int c, int per) { return ((c - 1 + per) % per) + 1; };
@@ -289,44 +304,36 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 const int ng = (mc + WAVE - 1) / WAVE;
                 TC cx[G][D];
                 if constexpr (!kLight) __syncthreads();  // the previous chunk's pairs are done
-                // Lanes past the chunk end hold +Inf (never accepted).
+                // Group gi holds elements m0 + 64 gi + lane: the piece of element m is the
+                // last u with pre[u] <= m, its sorted position m + rdel[u].  Lanes past the
+                // chunk end hold +Inf (never accepted).
 #pragma unroll
-                for (int gi = 0; gi < G; ++gi)
-                    if (gi == ng - 1) {
+                for (int gi = 0; gi < G; ++gi) {
+                    if (gi < ng) {
+                        const int ml = gi * WAVE + lane;
+                        const int m = m0 + ml;
+                        int del = rdel[0];
 #pragma unroll
-                        for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
-                    }
-                // Piece by piece: the groups a piece overlaps load its elements (m in the
-                // piece's range) from sorted position m + rdel[t].
+                        for (int t = 1; t < RP; ++t) del = m >= pre[t] ? rdel[t] : del;
+                        if (ml < mc) {
+                            const unsigned src = (unsigned)(m + del);
 #pragma unroll
-                for (int t = 0; t < RP; ++t) {
-                    const int lo = max(pre[t], m0), hi = min(pre[t + 1], m0 + mc);
-                    if (lo < hi) {
+                            for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
+                            if constexpr (!kLight) {
+                                Pack<TC, D> pk;
 #pragma unroll
-                        for (int gi = 0; gi < G; ++gi) {
-                            const int g_lo = m0 + gi * WAVE;
-                            if (g_lo < hi && g_lo + WAVE > lo) {
-                                const int m = g_lo + lane;
-                                if ((unsigned)(m - lo) < (unsigned)(hi - lo)) {
-                                    const unsigned src = (unsigned)(m + rdel[t]);
-                                    const int ml = gi * WAVE + lane;
+                                for (int d = 0; d < D; ++d) pk.v[d] = cx[gi][d];
+                                if constexpr (D == 3) pk.v[3] = (TC)0;
+                                s_c[ml] = pk;
+                                if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
+                                if constexpr (NF > 0) {
 #pragma unroll
-                                    for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
-                                    if constexpr (!kLight) {
-                                        Pack<TC, D> pk;
-#pragma unroll
-                                        for (int d = 0; d < D; ++d) pk.v[d] = cx[gi][d];
-                                        if constexpr (D == 3) pk.v[3] = (TC)0;
-                                        s_c[ml] = pk;
-                                        if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
-                                        if constexpr (NF > 0) {
-#pragma unroll
-                                            for (int fi = 0; fi < NF; ++fi)
-                                                s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
-                                        }
-                                    }
+                                    for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
                                 }
                             }
+                        } else {
+#pragma unroll
+                            for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
                         }
                     }
                 }
@@ -342,48 +349,52 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                         const TR p2 = pd[d] * pd[d];
                         d2 = d2 + p2;
                     }
-                    if (g.periodic) d2 = periodic_fix(pd, d2);
+                    if constexpr (PER) d2 = periodic_fix(pd, d2);
                     return d2 <= r2;
                 };
 
                 if constexpr (kLight) {
-                    for (int i = 0; i < nq; ++i) {
-                        TC xs[D];
+                    with_groups<G>(ng, [&](auto ngc) {
+                        constexpr int NG = decltype(ngc)::value;
+                        for (int i = 0; i < nq; ++i) {
+                            TC xs[D];
 #pragma unroll
-                        for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
-                        int c = 0;
+                            for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+                            int c = 0;
 #pragma unroll
-                        for (int gi = 0; gi < G; ++gi)
-                            if (gi < ng) c += __builtin_popcountll(__builtin_amdgcn_ballot_w64(test(xs, gi)));
-                        mycount += lane == i ? c : 0;
-                    }
+                            for (int gi = 0; gi < NG; ++gi)
+                                c += __builtin_popcountll(__builtin_amdgcn_ballot_w64(test(xs, gi)));
+                            mycount += lane == i ? c : 0;
+                        }
+                    });
                 } else {
                     __syncthreads();  // staged candidates visible to every lane
                     int P = 0, i0 = 0;
-                    for (int i = 0; i < nq; ++i) {
-                        if (P + ng * WAVE > PAIR_CAP) {
-                            flush(i0, i, P);
-                            i0 = i;
-                            P = 0;
-                        }
-                        if (lane == 0) s_qstart[i - i0] = P;
-                        TC xs[D];
+                    with_groups<G>(ng, [&](auto ngc) {
+                        constexpr int NG = decltype(ngc)::value;
+                        for (int i = 0; i < nq; ++i) {
+                            if (P + NG * WAVE > PAIR_CAP) {
+                                flush(i0, i, P);
+                                i0 = i;
+                                P = 0;
+                            }
+                            if (lane == 0) s_qstart[i - i0] = P;
+                            TC xs[D];
 #pragma unroll
-                        for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+                            for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
 #pragma unroll
-                        for (int gi = 0; gi < G; ++gi) {
-                            if (gi < ng) {
+                            for (int gi = 0; gi < NG; ++gi) {
+                                // accepted lanes take the next list slots in lane order; the
+                                // others write their own discard slot (no branch)
                                 const bool acc = test(xs, gi);
                                 const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
-                                if (mask) {
-                                    const int below = __builtin_amdgcn_mbcnt_hi(
-                                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                                    if (acc) s_pairs[P + below] = (uint16_t)(gi * WAVE + lane);
-                                    P += __builtin_popcountll(mask);
-                                }
+                                const int below = __builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                                s_pairs[acc ? P + below : PAIR_CAP + lane] = (uint16_t)(gi * WAVE + lane);
+                                P += __builtin_popcountll(mask);
                             }
                         }
-                    }
+                    });
                     flush(i0, nq, P);
                 }
             }

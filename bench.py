@@ -169,26 +169,34 @@ def main():
     extra = {}
     if args.workload == "count":
         trav_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-        kernel = "k_traverse<3,float,float,CountF>"
+        kernel = "CountF"
         algo = traversal_bytes(n, n_nonempty)
         extra["count_traversal_ms"] = trav_ms
     else:
         dens_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
         eos_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-        trav_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
-        kernel = "k_traverse<3,float,float,PressureAccelF>"
-        # per neighbour: coordinates + m_j + P_j; per query: P_i; out: 3 floats
-        algo = traversal_bytes(n, n_nonempty, f_in=8, f_own=4, f_out=12)
-        extra.update({"density_traversal_ms": dens_ms, "eos_ms": eos_ms, "pressure_traversal_ms": trav_ms,
+        pres_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+        extra.update({"density_traversal_ms": dens_ms, "eos_ms": eos_ms, "pressure_traversal_ms": pres_ms,
                       "density_pairs_per_s": pairs / (dens_ms * 1e-3),
-                      "pressure_pairs_per_s": pairs / (trav_ms * 1e-3)})
+                      "pressure_pairs_per_s": pairs / (pres_ms * 1e-3)})
+        # the dominant (slower) traversal of the step
+        if pres_ms >= dens_ms:
+            trav_ms, kernel = pres_ms, "PressureAccelF"
+            # per neighbour: coordinates + m_j + P_j; per query: P_i; out: 3 floats
+            algo = traversal_bytes(n, n_nonempty, f_in=8, f_own=4, f_out=12)
+        else:
+            trav_ms, kernel = dens_ms, "DensityF"
+            # per neighbour: coordinates + m_j; out: rho_i
+            algo = traversal_bytes(n, n_nonempty, f_in=4, f_own=0, f_out=4)
+    kernel = f"k_cells<3,float,float,{kernel}>"
     achieved = algo / (trav_ms * 1e-3) / 1e9
 
+    # measured HBM bytes per launch of this kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_1m.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            traffic = json.load(fh).get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -222,6 +230,7 @@ def main():
                    "nonempty_cells": int(n_nonempty), "max_cell_population": int(max_count)},
         "pairs_per_step": pairs,
         "dominant_traversal_ms": trav_ms,
+        "dominant_kernel": kernel,
         **extra,
         "wall_s": wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -235,7 +235,15 @@ int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype
  *   "algorithm"        0: candidate-major cells (default), 1: per-query candidate slices
  *   "persist_blocks"   workgroups of the persistent traversal grid (multiple of 8)
  *   "chunk"            candidates per chunk: -1 per functor, 0 small, 1 large
- *   "canonical_order"  1: in-cell order by original index; 0: atomic order */
+ *   "canonical_order"  1: in-cell order by original index; 0: atomic order
+ *   "pair_cache"       1 (grids without a periodic box): the first SPH density / pressure /
+ *                      pair-energy traversal of
+ *                      the grid's own points after a build records each cell's accepted pairs
+ *                      (about 2 bytes per pair in device memory); later ones on the same build
+ *                      replay them instead of testing candidates, with identical results.
+ *                      0 (default): always test.  The adjoints always recompute the traversal.
+ *   "pair_cache_per_point"  cache capacity in 2-byte entries per point (0: 108 in 3D, 36 in
+ *                      2D, 12 in 1D); cells that do not fit are traversed in full */
 int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value);
 
 /* Adds U[-amplitude, amplitude] to each of the n values of x (a pure function of seed,

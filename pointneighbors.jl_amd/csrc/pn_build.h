@@ -86,7 +86,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 // Build kernels
 // ------------------------------------------------------------------------------------
 
-__global__ void k_reset_status(DevStatus* st) {
+static __global__ void k_reset_status(DevStatus* st) {
     st->first_bad = ~0ull;
     st->total64 = 0ull;
     st->max_count = 0;
@@ -95,7 +95,7 @@ __global__ void k_reset_status(DevStatus* st) {
     st->pad = 0;
 }
 
-__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+static __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -228,7 +228,7 @@ __device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long
     return off + inc - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t C,
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32_t* __restrict__ counts, int64_t C,
                                                               unsigned long long* __restrict__ block_sums,
                                                               DevStatus* st) {
     __shared__ unsigned long long sh[SCAN_THREADS / WAVE];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32_t* __r
 }
 
 // Single workgroup: exclusive scan of the block sums, writes the totals.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long long* __restrict__ block_sums,
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long long* __restrict__ block_sums,
                                                            int64_t nblocks, int32_t* __restrict__ start,
                                                            int64_t C, DevStatus* st) {
     __shared__ unsigned long long sh[SCAN_THREADS / WAVE];
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long long* _
     }
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t* __restrict__ counts, int64_t C,
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t* __restrict__ counts, int64_t C,
                                                             const unsigned long long* __restrict__ block_sums,
                                                             int32_t* __restrict__ start,
                                                             int32_t* __restrict__ nonempty) {
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t* __res
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ keys,
+static __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ keys,
                                                  const int32_t* __restrict__ slots, int64_t sel_first,
                                                  const int32_t* __restrict__ sel_list, int64_t n_sel,
                                                  const int32_t* __restrict__ start,
@@ -383,6 +383,6 @@ __global__ __launch_bounds__(256) void k_displace(T* __restrict__ x, int64_t n, 
     x[i] += (T)((2.0 * u - 1.0) * (double)a);
 }
 
-__global__ void k_step_increment(int64_t* step) { *step += 1; }
+static __global__ void k_step_increment(int64_t* step) { *step += 1; }
 
 }  // namespace pn

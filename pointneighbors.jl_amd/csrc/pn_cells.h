@@ -29,6 +29,21 @@ namespace pn {
 // Pair-list capacity (entries) of the candidate-major traversal.
 constexpr int PAIR_CAP = 1024;
 
+// Pair cache: the first traversal of a cacheable functor after a build records, per
+// non-empty cell, its flushes; later traversals on the same build replay them instead of
+// testing candidates.  Each wavefront appends the records of its cells to its own region
+// of `data` (no atomics); a cell's records are contiguous:
+//     record = [batch, chunk, i0, i1, qstart[0 .. i1-i0], pair[0 .. P-1]]   (P = qstart[i1-i0])
+// head[2w] = first entry / 4 of work item w's records (-1: not cached, the replay runs
+// the full traversal for it), head[2w+1] = their length.  A replay loads a cell's records into
+// LDS with one wide copy and runs the recorded flushes, so results equal a full traversal.
+constexpr int PC_CELL_MAX = PAIR_CAP + WAVE;  // entries of one cell's records (the LDS list)
+struct PairCache {
+    uint16_t* data;
+    long long region;  // entries per wavefront (multiple of 4)
+    int* head;
+};
+
 template <class T>
 __device__ __forceinline__ T lane_bcast(T v, int i) {
     if constexpr (sizeof(T) == 4) {
@@ -63,8 +78,11 @@ __device__ __forceinline__ void with_groups(int n, Fn&& fn) {
     }
 }
 
-template <int D, class TC, class TR, class F, int G, bool PER>
-__global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, TR r2, F f, int nblocks) {
+// PCM: pair cache mode of this instance (0 off, 1 record, 2 replay); only functors with
+// kCache use 1 and 2.
+template <int D, class TC, class TR, class F, int G, bool PER, int PCM>
+__global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, TR r2, F f, int nblocks,
+                                                PairCache pc) {
     constexpr int R = Rows<D>::R;
     constexpr int RP = PER ? 2 * R : R;  // stencil row pieces
     constexpr int NF = F::kFields;
@@ -77,8 +95,10 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ Pack<TC, D> s_c[LC];
     __shared__ int s_j[(!kLight && F::kIndex) ? CH : 1];
     __shared__ TR s_f[NF > 0 ? NF : 1][(!kLight && NF > 0) ? CH : 1];
-    __shared__ uint16_t s_pairs[kLight ? 1 : PAIR_CAP + WAVE];  // + one discard slot per lane
-    __shared__ int s_qstart[LQ + 1];
+    // pair list (+ one discard slot per lane); a replay copies a cell's records here
+    __shared__ uint64_t s_pairs64[kLight ? 1 : PC_CELL_MAX / 4];
+    uint16_t* const s_pairs = (uint16_t*)s_pairs64;
+    __shared__ uint16_t s_qstart[LQ + 1];
     __shared__ Pack<TC, D> s_qc[LQ];
     __shared__ QS s_qst[LQ];
     __shared__ Acc s_acc[LQ];
@@ -121,33 +141,58 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         return d2;
     };
 
-    // Runs the pair body over pairs [0, P) of the list, which hold queries i0 .. i1-1
-    // (s_qstart[q - i0] = first pair of query q).  Lane t takes pairs
+    // Record mode: write position in this wavefront's region, start of the current cell's
+    // records, and whether they all fitted.
+    long long rec_pos = (long long)blockIdx.x * pc.region;
+    const long long rec_end = rec_pos + pc.region;
+    long long cell_base = rec_pos;
+    bool cell_ok = true;
+
+    // Runs the pair body over pairs [0, P) of list `prp`, which hold queries i0 .. i1-1
+    // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Lane t takes pairs
     // [t*P/64, (t+1)*P/64).  A query inside one lane's range is added to s_acc by that
     // lane; a query spanning lanes is summed by the lane holding its first pair, which
     // adds the following lanes' partial sums (s_con) in lane order.
-    auto flush = [&](int i0, int i1, int P) {
+    auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const uint16_t* prp) {
         const int nf = i1 - i0;
-        if (lane == 0) s_qstart[nf] = P;
         s_con[lane] = f.init();
         __syncthreads();
+        if constexpr (F::kCache && PCM == 1) {
+            if (cell_ok && P > 0) {
+                const long long size = 5 + nf + P;
+                if (rec_pos + size > rec_end || rec_pos + size - cell_base > PC_CELL_MAX) {
+                    cell_ok = false;
+                } else {
+                    uint16_t* rec = pc.data + rec_pos;
+                    if (lane == 0) {
+                        rec[0] = (uint16_t)batch;
+                        rec[1] = (uint16_t)chunk;
+                        rec[2] = (uint16_t)i0;
+                        rec[3] = (uint16_t)i1;
+                    }
+                    for (int t = lane; t <= nf; t += WAVE) rec[4 + t] = qsp[t];
+                    for (int k = lane; k < P; k += WAVE) rec[5 + nf + k] = prp[k];
+                    rec_pos += size;
+                }
+            }
+        }
         const int kb = (lane * P) >> 6, ke = ((lane + 1) * P) >> 6;
         int own = -1;
         Acc own_acc = f.init();
         if (kb < ke) {
-            int lo = 0, hi = nf;  // largest fq with s_qstart[fq] <= kb
+            int lo = 0, hi = nf;  // largest fq with qsp[fq] <= kb
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (s_qstart[mid] <= kb) lo = mid;
+                if (qsp[mid] <= kb) lo = mid;
                 else hi = mid;
             }
             int fq = lo;
-            int next = s_qstart[fq + 1];
+            int next = qsp[fq + 1];
             Pack<TC, D> qp = s_qc[i0 + fq];
             QS qst = s_qst[i0 + fq];
             Acc a = f.init();
             auto finish = [&]() {
-                const bool starts = s_qstart[fq] >= kb;
+                const bool starts = qsp[fq] >= kb;
                 if (starts && next <= ke) {
                     Acc t = s_acc[i0 + fq];
                     f.combine(t, a);
@@ -164,13 +209,13 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     finish();
                     do {
                         ++fq;
-                        next = s_qstart[fq + 1];
+                        next = qsp[fq + 1];
                     } while (k >= next);
                     qp = s_qc[i0 + fq];
                     qst = s_qst[i0 + fq];
                     a = f.init();
                 }
-                const int m = s_pairs[k];
+                const int m = prp[k];
                 const Pack<TC, D> pk = s_c[m];
                 TR pd[D];
 #pragma unroll
@@ -183,7 +228,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 }
                 if constexpr (PER) d2 = periodic_fix(pd, d2);
                 if constexpr (F::kFill) {
-                    const int slot = qst.rb + s_written[i0 + fq] + (k - s_qstart[fq]);
+                    const int slot = qst.rb + s_written[i0 + fq] + (k - (int)qsp[fq]);
                     if (slot < qst.re) f.col[slot] = s_j[m] + f.base;
                 } else {
                     int j = 0;
@@ -200,14 +245,14 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         }
         __syncthreads();
         if (own >= 0) {
-            const int e = s_qstart[own + 1];
+            const int e = qsp[own + 1];
             for (int t = lane + 1; t < WAVE && ((t * P) >> 6) < e; ++t) f.combine(own_acc, s_con[t]);
             Acc t = s_acc[i0 + own];
             f.combine(t, own_acc);
             s_acc[i0 + own] = t;
         }
         if constexpr (F::kFill) {
-            if (lane < nf) s_written[i0 + lane] += s_qstart[lane + 1] - s_qstart[lane];
+            if (lane < nf) s_written[i0 + lane] += qsp[lane + 1] - qsp[lane];
         }
         __syncthreads();
     };
@@ -223,6 +268,9 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     const int wchunk = (n_work + 7) >> 3;
     const int w_end = min(n_work, (xcd + 1) * wchunk);
     for (int w = xcd * wchunk + (int)(blockIdx.x >> 3); w < w_end; w += per_xcd) {
+        cell_base = (rec_pos + 3) & ~3ll;  // 8-byte aligned cell records
+        rec_pos = cell_base;
+        cell_ok = true;
         const int qc = Q.nonempty[w];
         const int qs = Q.start[qc], qe = Q.start[qc + 1];
         int n0 = g.n[0], n1 = g.n[1];
@@ -268,23 +316,101 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 }
             }
         }
-        int rdel[RP], pre[RP + 1];  // sorted position of element m of piece u: m + rdel[u]
-        pre[0] = 0;
+        // Piece table in lanes: lane u < RP holds piece u's first sorted position (rb), its
+        // length (rlen) and its first element index pre_v = rlen of lanes < u, summed.
+        int pre_v = rlen;
 #pragma unroll
-        for (int t = 0; t < RP; ++t) {
-            rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
-            pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(pre_v, o, WAVE);
+            if (lane >= o) pre_v += t;
         }
-        const int M = pre[RP];
+        const int M = __builtin_amdgcn_readlane(pre_v, RP - 1);
+        pre_v -= rlen;
+        // Counting keeps the piece table in scalar registers: element m of piece u sits at
+        // sorted position m + rdel[u].
+        int rdel[kLight ? RP : 1], pre[kLight ? RP + 1 : 1];
+        if constexpr (kLight) {
+            pre[0] = 0;
+#pragma unroll
+            for (int t = 0; t < RP; ++t) {
+                rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
+                pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
+            }
+        }
 
-        for (int q0 = qs; q0 < qe; q0 += WAVE) {
-            const int nq = min(WAVE, qe - q0);
+        // Counting: loads chunk m0 (mc elements) into registers; group gi holds elements
+        // m0 + 64 gi + lane, the piece of element m is the last u with pre[u] <= m.  Lanes
+        // past the chunk end hold +Inf (never accepted).
+        auto stage = [&](int m0, int mc, TC (&cx)[G][D]) {
+            const int ng = (mc + WAVE - 1) / WAVE;
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) {
+                if (gi < ng) {
+                    const int ml = gi * WAVE + lane;
+                    const int m = m0 + ml;
+                    int del = rdel[0];
+#pragma unroll
+                    for (int t = 1; t < RP; ++t) del = m >= pre[t] ? rdel[t] : del;
+                    if (ml < mc) {
+                        const unsigned src = (unsigned)(m + del);
+#pragma unroll
+                        for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
+                    }
+                }
+            }
+        };
+
+        // Pair mode: stages chunk m0 (mc elements: coordinates, indices, fields) in LDS piece
+        // by piece, each piece a contiguous range of the sorted order.
+        auto stage_lds = [&](int m0, int mc) {
+            __syncthreads();  // the previous chunk's pairs are done
+            for (int t = 0; t < RP; ++t) {
+                const int lo = __builtin_amdgcn_readlane(pre_v, t);
+                const int len = __builtin_amdgcn_readlane(rlen, t);
+                const int a = max(lo, m0), b = min(lo + len, m0 + mc);
+                if (a >= b) continue;
+                const int del = __builtin_amdgcn_readlane(rb, t) - lo;
+                for (int m = a + lane; m < b; m += WAVE) {
+                    const unsigned src = (unsigned)(m + del);
+                    const int ml = m - m0;
+                    Pack<TC, D> pk;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pk.v[d] = gc[(int64_t)d * Gs.cap + src];
+                    if constexpr (D == 3) pk.v[3] = (TC)0;
+                    s_c[ml] = pk;
+                    if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
+                    if constexpr (NF > 0) {
+#pragma unroll
+                        for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
+                    }
+                }
+            }
+            __syncthreads();
+        };
+
+        // Pair mode: the test registers of chunk (mc elements) from LDS.
+        auto load_regs = [&](int mc, TC (&cx)[G][D]) {
+            const int ng = (mc + WAVE - 1) / WAVE;
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) {
+                if (gi < ng) {
+                    const int ml = gi * WAVE + lane;
+                    const Pack<TC, D> pk = s_c[min(ml, CH - 1)];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) cx[gi][d] = ml < mc ? pk.v[d] : (TC)INFINITY;
+                }
+            }
+        };
+
+        // Query batch of up to 64 points starting at sorted position q0.
+        auto begin_batch = [&](int q0, int nq, TC (&xq)[D], int64_t& orig_i) {
             const bool qact = lane < nq;
-            TC xq[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) xq[d] = qact ? qcrd[(int64_t)d * Q.cap + q0 + lane] : (TC)0;
-            const int64_t orig_i = qact ? Q.sorted_idx[q0 + lane] : 0;
-            int mycount = 0;
+            orig_i = qact ? Q.sorted_idx[q0 + lane] : 0;
             if constexpr (!kLight) {
                 __syncthreads();
                 if (qact) {
@@ -298,44 +424,62 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     if constexpr (F::kFill) s_written[lane] = 0;
                 }
             }
+        };
 
-            for (int m0 = 0; m0 < M; m0 += CH) {
+        int rbase = -1, rsize = 0;
+        if constexpr (F::kCache && PCM == 2) {
+            rbase = pc.head[2 * (int64_t)w];
+            rsize = pc.head[2 * (int64_t)w + 1];
+        }
+        if (PCM == 2 && rbase >= 0) {
+            // replay: copy the cell's records to LDS (8 bytes per lane and step), then run
+            // the recorded flushes batch by batch
+            __syncthreads();
+            const uint64_t* src = (const uint64_t*)pc.data + rbase;
+            for (int k = lane; k < (rsize + 3) / 4; k += WAVE) s_pairs64[k] = src[k];
+            __syncthreads();
+            int pos = 0;
+            for (int q0 = qs, b = 0; q0 < qe; q0 += WAVE, ++b) {
+                const int nq = min(WAVE, qe - q0);
+                TC xq[D];
+                int64_t orig_i;
+                begin_batch(q0, nq, xq, orig_i);
+                int staged = -1;
+                while (pos < rsize && (int)s_pairs[pos] == b) {
+                    const int chunk = s_pairs[pos + 1], i0 = s_pairs[pos + 2], i1 = s_pairs[pos + 3];
+                    const int nf = i1 - i0;
+                    const int P = s_pairs[pos + 4 + nf];
+                    if (chunk != staged) {
+                        const int m0 = chunk * CH;
+                        stage_lds(m0, min(CH, M - m0));
+                        staged = chunk;
+                    }
+                    flush(i0, i1, P, b, chunk, s_pairs + pos + 4, s_pairs + pos + 5 + nf);
+                    pos += 5 + nf + P;
+                }
+                __syncthreads();
+                if (lane < nq) f.write(orig_i, s_qst[lane], s_acc[lane]);
+            }
+            continue;
+        }
+
+        for (int q0 = qs, b = 0; q0 < qe; q0 += WAVE, ++b) {
+            const int nq = min(WAVE, qe - q0);
+            const bool qact = lane < nq;
+            TC xq[D];
+            int64_t orig_i;
+            begin_batch(q0, nq, xq, orig_i);
+            int mycount = 0;
+
+            for (int m0 = 0, chunk = 0; m0 < M; m0 += CH, ++chunk) {
                 const int mc = min(CH, M - m0);
                 const int ng = (mc + WAVE - 1) / WAVE;
                 TC cx[G][D];
-                if constexpr (!kLight) __syncthreads();  // the previous chunk's pairs are done
-                // Group gi holds elements m0 + 64 gi + lane: the piece of element m is the
-                // last u with pre[u] <= m, its sorted position m + rdel[u].  Lanes past the
-                // chunk end hold +Inf (never accepted).
-#pragma unroll
-                for (int gi = 0; gi < G; ++gi) {
-                    if (gi < ng) {
-                        const int ml = gi * WAVE + lane;
-                        const int m = m0 + ml;
-                        int del = rdel[0];
-#pragma unroll
-                        for (int t = 1; t < RP; ++t) del = m >= pre[t] ? rdel[t] : del;
-                        if (ml < mc) {
-                            const unsigned src = (unsigned)(m + del);
-#pragma unroll
-                            for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
-                            if constexpr (!kLight) {
-                                Pack<TC, D> pk;
-#pragma unroll
-                                for (int d = 0; d < D; ++d) pk.v[d] = cx[gi][d];
-                                if constexpr (D == 3) pk.v[3] = (TC)0;
-                                s_c[ml] = pk;
-                                if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
-                                if constexpr (NF > 0) {
-#pragma unroll
-                                    for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
-                                }
-                            }
-                        } else {
-#pragma unroll
-                            for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
-                        }
-                    }
+                if constexpr (kLight) {
+                    stage(m0, mc, cx);
+                } else {
+                    stage_lds(m0, mc);
+                    load_regs(mc, cx);
                 }
 
                 // d2 of broadcast query xs against group gi, in the reference's arithmetic
@@ -368,17 +512,17 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                         }
                     });
                 } else {
-                    __syncthreads();  // staged candidates visible to every lane
                     int P = 0, i0 = 0;
                     with_groups<G>(ng, [&](auto ngc) {
                         constexpr int NG = decltype(ngc)::value;
                         for (int i = 0; i < nq; ++i) {
                             if (P + NG * WAVE > PAIR_CAP) {
-                                flush(i0, i, P);
+                                if (lane == 0) s_qstart[i - i0] = (uint16_t)P;
+                                flush(i0, i, P, b, chunk, s_qstart, s_pairs);
                                 i0 = i;
                                 P = 0;
                             }
-                            if (lane == 0) s_qstart[i - i0] = P;
+                            if (lane == 0) s_qstart[i - i0] = (uint16_t)P;
                             TC xs[D];
 #pragma unroll
                             for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
@@ -395,7 +539,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                             }
                         }
                     });
-                    flush(i0, nq, P);
+                    if (lane == 0) s_qstart[nq - i0] = (uint16_t)P;
+                    flush(i0, nq, P, b, chunk, s_qstart, s_pairs);
                 }
             }
             if constexpr (kLight) {
@@ -403,6 +548,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             } else {
                 __syncthreads();
                 if (qact) f.write(orig_i, s_qst[lane], s_acc[lane]);
+            }
+        }
+        if constexpr (F::kCache && PCM == 1) {
+            {
+                if (lane == 0) {
+                    pc.head[2 * (int64_t)w] = cell_ok ? (int)(cell_base >> 2) : -1;
+                    pc.head[2 * (int64_t)w + 1] = (int)(rec_pos - cell_base);
+                }
+                if (!cell_ok) rec_pos = cell_base;
             }
         }
     }

@@ -9,7 +9,7 @@ namespace pn {
 // Bitonic sort of each CSR row in LDS, one wavefront per row, rows up to SORT_CAP.
 constexpr int SORT_CAP = 4096;
 
-__global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ row_ptr, int64_t n_rows,
+static __global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ row_ptr, int64_t n_rows,
                                                     int32_t* __restrict__ col, int nblocks) {
     __shared__ int32_t s[SORT_CAP];
     const int lane = threadIdx.x;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ 
 // Precomputed-list traversal (src/nhs_precomputed.jl:210-247): iterate the stored list
 // of each selected row; no radius test.
 // ------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_list_count(const int32_t* __restrict__ row_ptr, int64_t sel_first,
+static __global__ __launch_bounds__(256) void k_list_count(const int32_t* __restrict__ row_ptr, int64_t sel_first,
                                                     const int32_t* __restrict__ sel_list, int64_t n_sel,
                                                     int32_t* __restrict__ out, int accumulate) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_list_count(const int32_t* __restrict__ 
 
 // Sum of (stored value - stored_base + index_base) over each selected row; one
 // wavefront per row, coalesced over the row.
-__global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* __restrict__ row_ptr,
+static __global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* __restrict__ row_ptr,
                                                          const int32_t* __restrict__ col, int64_t sel_first,
                                                          const int32_t* __restrict__ sel_list, int64_t n_sel,
                                                          long long shift, int64_t* __restrict__ out,

@@ -107,6 +107,7 @@ template <class TR, int D>
 struct DensityF : FunctorBase {
     using Acc = TR;
     static constexpr bool kQueue = true;
+    static constexpr bool kCache = true;
     static constexpr int kFields = 1;
     SmoothingKernel<TR, D> kern;
     TR* rho;
@@ -125,6 +126,7 @@ template <class TR, int D>
 struct PressureAccelF : FunctorBase {
     using Acc = Vec<TR, D>;
     static constexpr bool kQueue = true;
+    static constexpr bool kCache = true;
     struct QState {
         TR P;
     };
@@ -164,6 +166,7 @@ template <class TR, int D>
 struct EnergyF : FunctorBase {
     using Acc = TR;
     static constexpr bool kQueue = true;
+    static constexpr bool kCache = true;
     TR eps, inv_rc;
     TR* e;
     int accumulate;

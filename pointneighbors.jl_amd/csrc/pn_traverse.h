@@ -46,12 +46,15 @@ struct Vec {
 //                            runs in lockstep over the queues (for bodies much costlier
 //                            than the distance test; ~1 in 6 candidates is accepted)
 //   kLight                   counting only (no pair body)
+//   kCache                   forward pair sum whose pair lists may be recorded once per build
+//                            and replayed by later traversals (the adjoints recompute)
 struct FunctorBase {
     static constexpr bool kIndex = false;
     static constexpr int kFields = 0;
     static constexpr bool kFill = false;
     static constexpr bool kQueue = false;  // heavy pair body: run it on queued pairs only
     static constexpr bool kLight = false;  // counting only: no pair body
+    static constexpr bool kCache = false;  // forward pair sum: may replay a recorded pair list
     using QState = Empty;
     const void* fsorted = nullptr;
     int64_t fstride = 0;

@@ -394,3 +394,29 @@ def test_precomputed_lists_and_freeze(orc, golden):
         pn.initialize(p3, x, x)
     with pytest.raises(pn.PointNeighborsError, match="inactive points"):
         pn.initialize(p, x, x, eachindex_y=range(3, 10))
+
+
+@pytest.mark.parametrize("n_pts,extent,longest", [(900, 6.0, 256), (2600, 9.0, 512)])
+def test_precomputed_long_rows(orc, n_pts, extent, longest):
+    """Rows from a few to ~1,500 neighbours: every row-sort path (1, 2, 4, 8 keys per lane in
+    registers, LDS beyond 512) must give the oracle's sorted lists."""
+    rng = np.random.default_rng(n_pts)
+    # a dense core and a sparse halo: row lengths spread over the whole range
+    core = rng.normal(0.0, 0.12 * extent, size=(n_pts // 2, 3))
+    halo = rng.uniform(-extent, extent, size=(n_pts - n_pts // 2, 3))
+    pts = np.concatenate([core, halo]).astype(np.float64)
+    r = 0.3 * extent
+    lo, hi = pts.min(axis=0) - r, pts.max(axis=0) + r
+    inner = make_nhs(pts, r, lo, hi, cap=4096)
+    p = pn.PrecomputedNeighborhoodSearch(3, search_radius=r, n_points=len(pts),
+                                         update_neighborhood_search=inner, max_neighbors=4096)
+    x = dev(pts)
+    pn.initialize(p, x, x)
+    og = oracle_like(orc, inner, np.float64)
+    assert og.build(pts)[0] == orc.OK
+    st, _, (orp, ocols) = og.precomputed(pts, pts, 4096)
+    assert st == orc.OK
+    lens = np.diff(orp)
+    assert lens.max() > longest and (lens <= 64).any() and ((lens > 128) & (lens <= 256)).any()
+    rp, cols = p.neighbor_lists.to_host()
+    assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(cols, ocols)

@@ -368,7 +368,7 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
         for (int d = 0; d < ndims; ++d)
             if (!(box_size[d] > 0.0)) return fail(PN_ERR_ARGUMENT, "box sizes must be positive");
     if (sel_count <= 0) return PN_OK;
-    const int nblocks = (int)round_up8(sel_count);
+    const int nblocks = (int)round_up8((sel_count + pn::LIST_ROWS_PER_BLOCK - 1) / pn::LIST_ROWS_PER_BLOCK);
     pn_grid tmp;  // dispatch only
     tmp.nd = ndims;
     tmp.tc = coord_dtype;
@@ -383,7 +383,7 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
         const TR r = (TR)search_radius;
         per.r2 = r * r;
         for (int d = 0; d < 3; ++d) per.box[d] = (box_size && d < D) ? (TR)box_size[d] : (TR)1;
-        hipLaunchKernelGGL((pn::k_list_density<D, TC, TR, pn::SmoothingKernel<TR, D>>), dim3(nblocks), dim3(pn::WAVE),
+        hipLaunchKernelGGL((pn::k_list_density<D, TC, TR, pn::SmoothingKernel<TR, D>>), dim3(nblocks), dim3(256),
                            0, (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
                            (const TC*)x, (const TC*)y, (const TR*)mass, kern, (TR*)rho, accumulate, nblocks, per);
         PN_CHECK_LAUNCH();

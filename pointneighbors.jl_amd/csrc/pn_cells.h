@@ -176,6 +176,24 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 }
             }
         }
+        if constexpr (F::kFill) {
+            // CSR fill: pair k goes to lane k % 64, so a store instruction writes consecutive
+            // slots of one or two rows; each lane's query cursor only moves forward.
+            int fq = 0, nxt = qsp[1];
+            for (int k = lane; k < P; k += WAVE) {
+                while (k >= nxt) {
+                    ++fq;
+                    nxt = qsp[fq + 1];
+                }
+                const QS qst = s_qst[i0 + fq];
+                const int slot = qst.rb + s_written[i0 + fq] + (k - (int)qsp[fq]);
+                if (slot < qst.re) f.col[slot] = s_j[prp[k]] + f.base;
+            }
+            __syncthreads();
+            if (lane < nf) s_written[i0 + lane] += qsp[lane + 1] - qsp[lane];
+            __syncthreads();
+            return;
+        }
         const int kb = (lane * P) >> 6, ke = ((lane + 1) * P) >> 6;
         int own = -1;
         Acc own_acc = f.init();
@@ -227,10 +245,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     d2 = d2 + p2;
                 }
                 if constexpr (PER) d2 = periodic_fix(pd, d2);
-                if constexpr (F::kFill) {
-                    const int slot = qst.rb + s_written[i0 + fq] + (k - (int)qsp[fq]);
-                    if (slot < qst.re) f.col[slot] = s_j[m] + f.base;
-                } else {
+                {
                     int j = 0;
                     if constexpr (F::kIndex) j = s_j[m];
                     TR fj[NF > 0 ? NF : 1];
@@ -250,9 +265,6 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             Acc t = s_acc[i0 + own];
             f.combine(t, own_acc);
             s_acc[i0 + own] = t;
-        }
-        if constexpr (F::kFill) {
-            if (lane < nf) s_written[i0 + lane] += qsp[lane + 1] - qsp[lane];
         }
         __syncthreads();
     };

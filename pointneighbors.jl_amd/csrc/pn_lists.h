@@ -9,87 +9,132 @@ namespace pn {
 // Sorting of CSR rows (sorteach!, src/vector_of_vectors.jl:177-212), rows up to SORT_CAP.
 constexpr int SORT_CAP = 4096;
 
-// x from lane (lane ^ m), m in 1..32.  ds_swizzle's bit mode covers xor masks inside
-// 32-lane halves; 32 goes through ds_bpermute.
+// x from lane (lane ^ m), m in 1..32: DPP quad permutations for 1 and 2 (a VALU move),
+// ds_swizzle's bit mode for xor masks inside 32-lane halves, ds_bpermute for 32.
 template <int M>
 __device__ __forceinline__ int lane_xor(int x) {
-    if constexpr (M < 32) return __builtin_amdgcn_ds_swizzle(x, (M << 10) | 0x1F);
+    if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    else if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else if constexpr (M < 32) return __builtin_amdgcn_ds_swizzle(x, (M << 10) | 0x1F);
     else return __builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ M) << 2), x);
 }
 
-// One bitonic stage (sequence length SIZE, stride STRIDE) over the 64*K keys held as
-// v[j] = key (lane + 64 j): ascending overall.
-template <int K, int SIZE, int STRIDE>
-__device__ __forceinline__ void bitonic_stage(int (&v)[K], int lane) {
+// Median of three: with c = INT32_MIN it is min(a, b), with c = INT32_MAX it is max(a, b).
+__device__ __forceinline__ int med3_i32(int a, int b, int c) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// One bitonic stage (sequence length SIZE, stride STRIDE) over R rows of 64*K keys each,
+// v[r][j] = key (lane + 64 j) of row r: ascending overall.  A lane that keeps the smaller
+// key of its pair takes med3(own, partner, INT32_MIN), the other med3(.., INT32_MAX); the
+// selector depends on the lane only, so it stays in a register across rows.
+template <int K, int R, int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_stage(int (&v)[R][K], int lane) {
     if constexpr (STRIDE >= WAVE) {
         constexpr int JS = STRIDE / WAVE;
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            if ((j & JS) == 0) {
-                const bool asc = (((lane + WAVE * j) & SIZE) == 0);
-                const int a = v[j], b = v[j + JS];
-                v[j] = asc ? min(a, b) : max(a, b);
-                v[j + JS] = asc ? max(a, b) : min(a, b);
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if ((j & JS) == 0) {
+                    const bool asc = ((WAVE * j) & SIZE) == 0;
+                    const int a = v[r][j], b = v[r][j + JS];
+                    v[r][j] = asc ? min(a, b) : max(a, b);
+                    v[r][j + JS] = asc ? max(a, b) : min(a, b);
+                }
             }
         }
     } else {
         const bool lower = (lane & STRIDE) == 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const int p = lane_xor<STRIDE>(v[j]);
             const bool asc = (((lane + WAVE * j) & SIZE) == 0);
-            v[j] = (lower == asc) ? min(v[j], p) : max(v[j], p);
+            const int sel = (lower == asc) ? INT32_MIN : INT32_MAX;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r][j] = med3_i32(v[r][j], lane_xor<STRIDE>(v[r][j]), sel);
         }
     }
 }
 
-template <int K, int SIZE, int STRIDE>
-__device__ __forceinline__ void bitonic_strides(int (&v)[K], int lane) {
-    bitonic_stage<K, SIZE, STRIDE>(v, lane);
-    if constexpr (STRIDE > 1) bitonic_strides<K, SIZE, STRIDE / 2>(v, lane);
+template <int K, int R, int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_strides(int (&v)[R][K], int lane) {
+    bitonic_stage<K, R, SIZE, STRIDE>(v, lane);
+    if constexpr (STRIDE > 1) bitonic_strides<K, R, SIZE, STRIDE / 2>(v, lane);
 }
 
-template <int K, int SIZE>
-__device__ __forceinline__ void bitonic_sizes(int (&v)[K], int lane) {
-    bitonic_strides<K, SIZE, SIZE / 2>(v, lane);
-    if constexpr (SIZE < WAVE * K) bitonic_sizes<K, SIZE * 2>(v, lane);
+template <int K, int R, int SIZE>
+__device__ __forceinline__ void bitonic_sizes(int (&v)[R][K], int lane) {
+    bitonic_strides<K, R, SIZE, SIZE / 2>(v, lane);
+    if constexpr (SIZE < WAVE * K) bitonic_sizes<K, R, SIZE * 2>(v, lane);
 }
 
-// Sorts row [b, b + n), n <= 64 K, in registers (padding +INT32_MAX sorts last).
+// Sorts R rows [b[r], b[r] + n[r]), each n[r] <= 64 K, in registers (padding INT32_MAX sorts
+// last); the R rows' networks are independent, so their latencies overlap.
+template <int K, int R>
+__device__ __forceinline__ void sort_rows_regs(int32_t* __restrict__ col, const int (&b)[R], const int (&n)[R],
+                                               int lane) {
+    int v[R][K];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int k = lane + WAVE * j;
+            v[r][j] = k < n[r] ? col[b[r] + k] : INT32_MAX;
+        }
+    }
+    bitonic_sizes<K, R, 2>(v, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int k = lane + WAVE * j;
+            if (k < n[r]) col[b[r] + k] = v[r][j];
+        }
+    }
+}
+
 template <int K>
 __device__ __forceinline__ void sort_row_regs(int32_t* __restrict__ col, int b, int n, int lane) {
-    int v[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        const int k = lane + WAVE * j;
-        v[j] = k < n ? col[b + k] : INT32_MAX;
-    }
-    bitonic_sizes<K, 2>(v, lane);
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        const int k = lane + WAVE * j;
-        if (k < n) col[b + k] = v[j];
-    }
+    const int bb[1] = {b}, nn[1] = {n};
+    sort_rows_regs<K, 1>(col, bb, nn, lane);
 }
 
-// A persistent grid of wavefronts walks the rows (XCD-contiguous ranges) and sorts rows of
-// up to 512 entries in registers; k_sort_rows_lds sorts the longer ones (up to SORT_CAP).
+// A persistent grid of wavefronts walks the rows (XCD-contiguous ranges, two rows per step)
+// and sorts rows of up to 512 entries in registers; k_sort_rows_lds sorts the longer ones
+// (up to SORT_CAP).
 constexpr int SORT_REGS_MAX = 8 * WAVE;
 
 static __global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ row_ptr, int64_t n_rows,
                                                     int32_t* __restrict__ col, int nblocks) {
     const int lane = threadIdx.x;
     const int xcd = blockIdx.x & 7, per_xcd = nblocks >> 3;
-    const int64_t chunk = (n_rows + 7) >> 3;
+    const int64_t chunk = (((n_rows + 7) >> 3) + 1) & ~int64_t(1);  // even: row pairs stay in one range
     const int64_t r_end = min(n_rows, (int64_t)(xcd + 1) * chunk);
-    for (int64_t row = (int64_t)xcd * chunk + (blockIdx.x >> 3); row < r_end; row += per_xcd) {
-        const int b = row_ptr[row], e = row_ptr[row + 1];
-        const int n = e - b;
-        if (n <= 1 || n > SORT_REGS_MAX) continue;
-        if (n <= WAVE) sort_row_regs<1>(col, b, n, lane);
-        else if (n <= 2 * WAVE) sort_row_regs<2>(col, b, n, lane);
-        else if (n <= 4 * WAVE) sort_row_regs<4>(col, b, n, lane);
-        else sort_row_regs<8>(col, b, n, lane);
+    for (int64_t row = (int64_t)xcd * chunk + 2 * (int64_t)(blockIdx.x >> 3); row < r_end; row += 2 * per_xcd) {
+        const bool two = row + 1 < r_end;
+        const int b0 = row_ptr[row], e0 = row_ptr[row + 1];
+        const int e1 = two ? row_ptr[row + 2] : e0;
+        const int n0 = e0 - b0, n1 = e1 - e0;
+        const int m = max(n0, n1);
+        if (m <= 2 * WAVE) {
+            // both rows fit two registers per lane: sort them together
+            const int bb[2] = {b0, e0}, nn[2] = {n0 > 1 ? n0 : 0, n1 > 1 ? n1 : 0};
+            if (m <= WAVE) sort_rows_regs<1, 2>(col, bb, nn, lane);
+            else sort_rows_regs<2, 2>(col, bb, nn, lane);
+            continue;
+        }
+        const int bs[2] = {b0, e0}, ns[2] = {n0, n1};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int n = ns[r], b = bs[r];
+            if (n <= 1 || n > SORT_REGS_MAX) continue;
+            if (n <= WAVE) sort_row_regs<1>(col, b, n, lane);
+            else if (n <= 2 * WAVE) sort_row_regs<2>(col, b, n, lane);
+            else if (n <= 4 * WAVE) sort_row_regs<4>(col, b, n, lane);
+            else sort_row_regs<8>(col, b, n, lane);
+        }
     }
 }
 
@@ -169,55 +214,80 @@ struct ListPeriodic {
 
 // SPH density over stored lists (the TLSPH-style sum of a PrecomputedNeighborhoodSearch,
 // src/nhs_precomputed.jl:210-247: stored neighbours, distance recomputed, no radius
-// test).  One wavefront per selected row, lanes stride the row (coalesced column reads),
-// wave reduction in a fixed order.  x and y are D x N coordinate matrices (original
+// test).  Sixteen lanes per selected row (LIST_ROWS_PER_BLOCK rows per 256-thread block):
+// each lane first loads up to LIST_UNROLL of its row's column indices, then all their
+// coordinates and masses, so a row's loads are in flight together; the 16 partial sums are
+// combined in a fixed butterfly order.  x and y are D x N coordinate matrices (original
 // order), mass per y point, rho per x point.
+constexpr int LIST_LANES = 16;
+constexpr int LIST_ROWS_PER_BLOCK = 256 / LIST_LANES;
+constexpr int LIST_UNROLL = 8;
+
 template <int D, class TC, class TR, class K>
-__global__ __launch_bounds__(WAVE) void k_list_density(const int32_t* __restrict__ row_ptr,
-                                                       const int32_t* __restrict__ col, int stored_base,
-                                                       int64_t sel_first, const int32_t* __restrict__ sel_list,
-                                                       int64_t n_sel, const TC* __restrict__ x,
-                                                       const TC* __restrict__ y, const TR* __restrict__ mass,
-                                                       K kern, TR* __restrict__ rho, int accumulate, int nblocks,
-                                                       ListPeriodic<TR> per) {
-    const int64_t a = xcd_remap(blockIdx.x, nblocks);
-    if (a >= n_sel) return;
+__global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col, int stored_base,
+                                                      int64_t sel_first, const int32_t* __restrict__ sel_list,
+                                                      int64_t n_sel, const TC* __restrict__ x,
+                                                      const TC* __restrict__ y, const TR* __restrict__ mass,
+                                                      K kern, TR* __restrict__ rho, int accumulate, int nblocks,
+                                                      ListPeriodic<TR> per) {
+    const int sub = threadIdx.x & (LIST_LANES - 1);
+    const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
+    if (a >= n_sel) return;  // whole 16-lane groups leave together
     const int64_t i = sel_point(a, sel_first, sel_list);
     const int b = row_ptr[i], e = row_ptr[i + 1];
     TC xi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)D * i + d];
     TR s = (TR)0;
-    for (int k = b + (int)threadIdx.x; k < e; k += WAVE) {
-        const int64_t j = (int64_t)col[k] - stored_base;
-        TR pd[D];
+    for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * LIST_UNROLL) {
+        int64_t j[LIST_UNROLL];
 #pragma unroll
-        for (int d = 0; d < D; ++d) pd[d] = (TR)(xi[d] - y[(int64_t)D * j + d]);
-        TR d2 = pd[0] * pd[0];
-#pragma unroll
-        for (int d = 1; d < D; ++d) {
-            const TR p2 = pd[d] * pd[d];
-            d2 = d2 + p2;
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            const int k = k0 + LIST_LANES * u;
+            j[u] = k < e ? (int64_t)col[k] - stored_base : -1;
         }
-        if (per.on && d2 > per.r2) {
+        TC yv[LIST_UNROLL][D];
+        TR mv[LIST_UNROLL];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const TR kk = rint(pd[d] / per.box[d]);
-                const TR sh = per.box[d] * kk;
-                pd[d] = pd[d] - sh;
-            }
-            d2 = pd[0] * pd[0];
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            const int64_t jj = j[u] >= 0 ? j[u] : 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) yv[u][d] = y[(int64_t)D * jj + d];
+            mv[u] = mass[jj];
+        }
+#pragma unroll
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            TR pd[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) pd[d] = (TR)(xi[d] - yv[u][d]);
+            TR d2 = pd[0] * pd[0];
 #pragma unroll
             for (int d = 1; d < D; ++d) {
                 const TR p2 = pd[d] * pd[d];
                 d2 = d2 + p2;
             }
+            if (per.on && d2 > per.r2) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const TR kk = rint(pd[d] / per.box[d]);
+                    const TR sh = per.box[d] * kk;
+                    pd[d] = pd[d] - sh;
+                }
+                d2 = pd[0] * pd[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) {
+                    const TR p2 = pd[d] * pd[d];
+                    d2 = d2 + p2;
+                }
+            }
+            const TR t = mv[u] * kern.w(kern.sqrt_(d2));
+            s += j[u] >= 0 ? t : (TR)0;
         }
-        s += mass[j] * kern.w(kern.sqrt_(d2));
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
-    if (threadIdx.x == 0) rho[i] = accumulate ? rho[i] + s : s;
+    for (int o = LIST_LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LIST_LANES);
+    if (sub == 0) rho[i] = accumulate ? rho[i] + s : s;
 }
 
 }  // namespace pn

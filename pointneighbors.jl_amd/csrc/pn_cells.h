@@ -338,16 +338,14 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         }
         const int M = __builtin_amdgcn_readlane(pre_v, RP - 1);
         pre_v -= rlen;
-        // Counting keeps the piece table in scalar registers: element m of piece u sits at
-        // sorted position m + rdel[u].
-        int rdel[kLight ? RP : 1], pre[kLight ? RP + 1 : 1];
-        if constexpr (kLight) {
-            pre[0] = 0;
+        // The piece table in scalar registers: element m of piece u sits at sorted position
+        // m + rdel[u].
+        int rdel[RP], pre[RP + 1];
+        pre[0] = 0;
 #pragma unroll
-            for (int t = 0; t < RP; ++t) {
-                rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
-                pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
-            }
+        for (int t = 0; t < RP; ++t) {
+            rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
+            pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
         }
 
         // Counting: loads chunk m0 (mc elements) into registers; group gi holds elements
@@ -375,46 +373,59 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             }
         };
 
-        // Pair mode: stages chunk m0 (mc elements: coordinates, indices, fields) in LDS piece
-        // by piece, each piece a contiguous range of the sorted order.
-        auto stage_lds = [&](int m0, int mc) {
-            __syncthreads();  // the previous chunk's pairs are done
-            for (int t = 0; t < RP; ++t) {
-                const int lo = __builtin_amdgcn_readlane(pre_v, t);
-                const int len = __builtin_amdgcn_readlane(rlen, t);
-                const int a = max(lo, m0), b = min(lo + len, m0 + mc);
-                if (a >= b) continue;
-                const int del = __builtin_amdgcn_readlane(rb, t) - lo;
-                for (int m = a + lane; m < b; m += WAVE) {
-                    const unsigned src = (unsigned)(m + del);
-                    const int ml = m - m0;
-                    Pack<TC, D> pk;
+        // Pair mode: stages chunk m0 (mc elements: coordinates, indices, fields) in LDS.  Lane
+        // l gathers elements m0 + 64 gi + l of every group first (all loads in flight), then
+        // writes them; the test registers cx (lanes past the chunk: +Inf) come from the same
+        // loads.
+        auto stage_lds = [&](int m0, int mc, TC (&cx)[G][D]) {
+            const int ng = (mc + WAVE - 1) / WAVE;
+            Pack<TC, D> pk[G];
+            int jv[G];
+            TR fv[G][NF > 0 ? NF : 1];
 #pragma unroll
-                    for (int d = 0; d < D; ++d) pk.v[d] = gc[(int64_t)d * Gs.cap + src];
-                    if constexpr (D == 3) pk.v[3] = (TC)0;
-                    s_c[ml] = pk;
-                    if constexpr (F::kIndex) s_j[ml] = Gs.sorted_idx[src];
+            for (int gi = 0; gi < G; ++gi) {
+                const int ml = gi * WAVE + lane;
+#pragma unroll
+                for (int d = 0; d < (D == 3 ? 4 : D); ++d) pk[gi].v[d] = (TC)0;
+                jv[gi] = 0;
+#pragma unroll
+                for (int fi = 0; fi < (NF > 0 ? NF : 1); ++fi) fv[gi][fi] = (TR)0;
+                if (gi < ng && ml < mc) {
+                    const int m = m0 + ml;
+                    // a sum of per-piece steps (a select chain here becomes a scratch table load)
+                    int del = rdel[0];
+#pragma unroll
+                    for (int t = 1; t < RP; ++t) del += m >= pre[t] ? rdel[t] - rdel[t - 1] : 0;
+                    const unsigned src = (unsigned)(m + del);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pk[gi].v[d] = gc[(int64_t)d * Gs.cap + src];
+                    if constexpr (D == 3) pk[gi].v[3] = (TC)0;
+                    if constexpr (F::kIndex) jv[gi] = Gs.sorted_idx[src];
                     if constexpr (NF > 0) {
 #pragma unroll
-                        for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fs[(int64_t)fi * f.fstride + src];
+                        for (int fi = 0; fi < NF; ++fi) fv[gi][fi] = fs[(int64_t)fi * f.fstride + src];
                     }
                 }
             }
-            __syncthreads();
-        };
-
-        // Pair mode: the test registers of chunk (mc elements) from LDS.
-        auto load_regs = [&](int mc, TC (&cx)[G][D]) {
-            const int ng = (mc + WAVE - 1) / WAVE;
+            __syncthreads();  // the previous chunk's pairs are done
 #pragma unroll
             for (int gi = 0; gi < G; ++gi) {
+                const int ml = gi * WAVE + lane;
                 if (gi < ng) {
-                    const int ml = gi * WAVE + lane;
-                    const Pack<TC, D> pk = s_c[min(ml, CH - 1)];
+                    const bool in = ml < mc;
+                    if (in) {
+                        s_c[ml] = pk[gi];
+                        if constexpr (F::kIndex) s_j[ml] = jv[gi];
+                        if constexpr (NF > 0) {
 #pragma unroll
-                    for (int d = 0; d < D; ++d) cx[gi][d] = ml < mc ? pk.v[d] : (TC)INFINITY;
+                            for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fv[gi][fi];
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < D; ++d) cx[gi][d] = in ? pk[gi].v[d] : (TC)INFINITY;
                 }
             }
+            __syncthreads();
         };
 
         // Query batch of up to 64 points starting at sorted position q0.
@@ -463,7 +474,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     const int P = s_pairs[pos + 4 + nf];
                     if (chunk != staged) {
                         const int m0 = chunk * CH;
-                        stage_lds(m0, min(CH, M - m0));
+                        TC unused[G][D];
+                        stage_lds(m0, min(CH, M - m0), unused);
                         staged = chunk;
                     }
                     flush(i0, i1, P, b, chunk, s_pairs + pos + 4, s_pairs + pos + 5 + nf);
@@ -490,8 +502,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 if constexpr (kLight) {
                     stage(m0, mc, cx);
                 } else {
-                    stage_lds(m0, mc);
-                    load_regs(mc, cx);
+                    stage_lds(m0, mc, cx);
                 }
 
                 // d2 of broadcast query xs against group gi, in the reference's arithmetic

@@ -56,11 +56,13 @@ __device__ __forceinline__ T lane_bcast(T v, int i) {
     }
 }
 
-// Candidate groups held in registers per chunk.
+// Candidate groups held in registers per chunk.  Pair bodies: 3 groups (192 candidates) for
+// Float32 by default, which keeps LDS small enough for 4-5 wavefronts per SIMD; the large
+// variant holds a 3D stencil of ~14 points per cell in one chunk.
 template <class TC, class F>
 struct CellGroups {
     static constexpr int large = F::kLight ? 8 : (sizeof(TC) == 4 ? 6 : 4);
-    static constexpr int small = F::kLight ? 8 : (sizeof(TC) == 4 ? 4 : 2);
+    static constexpr int small = F::kLight ? 8 : (sizeof(TC) == 4 ? 3 : 2);
 };
 
 template <int N>
@@ -94,12 +96,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     constexpr int LQ = kLight ? 1 : WAVE;
     __shared__ Pack<TC, D> s_c[LC];
     __shared__ int s_j[(!kLight && F::kIndex) ? CH : 1];
-    __shared__ TR s_f[NF > 0 ? NF : 1][(!kLight && NF > 0) ? CH : 1];
+    // 3D: the first per-neighbour field rides in the padding slot of the candidate record
+    constexpr bool kPackF = D == 3 && NF > 0 && std::is_same<TC, TR>::value;
+    constexpr int FOFF = kPackF ? 1 : 0;
+    constexpr int NFL = NF - FOFF;  // fields kept in their own LDS planes
+    __shared__ TR s_f[NFL > 0 ? NFL : 1][(!kLight && NFL > 0) ? CH : 1];
     // pair list (+ one discard slot per lane); a replay copies a cell's records here
     __shared__ uint64_t s_pairs64[kLight ? 1 : PC_CELL_MAX / 4];
     uint16_t* const s_pairs = (uint16_t*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
-    __shared__ Pack<TC, D> s_qc[LQ];
     __shared__ QS s_qst[LQ];
     __shared__ Acc s_acc[LQ];
     __shared__ Acc s_con[LQ];
@@ -152,7 +157,16 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Lane t takes pairs
     // [t*P/64, (t+1)*P/64).  A query inside one lane's range is added to s_acc by that
     // lane; a query spanning lanes is summed by the lane holding its first pair, which
-    // adds the following lanes' partial sums (s_con) in lane order.
+    // adds the following lanes' partial sums (s_con) in lane order.  The pair bodies read a
+    // query's coordinates from its sorted plane (L1/L2-resident; keeping them out of LDS buys
+    // occupancy).
+    int cur_q0 = 0;  // sorted position of the current query batch
+    auto qpack = [&](int q) {
+        Pack<TC, D> pk;
+#pragma unroll
+        for (int d = 0; d < D; ++d) pk.v[d] = ((const TC*)Q.coords)[(int64_t)d * Q.cap + cur_q0 + q];
+        return pk;
+    };
     auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const uint16_t* prp) {
         const int nf = i1 - i0;
         s_con[lane] = f.init();
@@ -206,7 +220,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             }
             int fq = lo;
             int next = qsp[fq + 1];
-            Pack<TC, D> qp = s_qc[i0 + fq];
+            Pack<TC, D> qp = qpack(i0 + fq);
             QS qst = s_qst[i0 + fq];
             Acc a = f.init();
             auto finish = [&]() {
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                         ++fq;
                         next = qsp[fq + 1];
                     } while (k >= next);
-                    qp = s_qc[i0 + fq];
+                    qp = qpack(i0 + fq);
                     qst = s_qst[i0 + fq];
                     a = f.init();
                 }
@@ -249,9 +263,10 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     int j = 0;
                     if constexpr (F::kIndex) j = s_j[m];
                     TR fj[NF > 0 ? NF : 1];
-                    if constexpr (NF > 0) {
+                    if constexpr (kPackF) fj[0] = pk.v[3];
+                    if constexpr (NFL > 0) {
 #pragma unroll
-                        for (int fi = 0; fi < NF; ++fi) fj[fi] = s_f[fi][m];
+                        for (int fi = FOFF; fi < NF; ++fi) fj[fi] = s_f[fi - FOFF][m];
                     }
                     f.template pair<TR, D>(a, qst, pd, d2, j, fj, nullptr);
                 }
@@ -399,12 +414,12 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     const unsigned src = (unsigned)(m + del);
 #pragma unroll
                     for (int d = 0; d < D; ++d) pk[gi].v[d] = gc[(int64_t)d * Gs.cap + src];
-                    if constexpr (D == 3) pk[gi].v[3] = (TC)0;
                     if constexpr (F::kIndex) jv[gi] = Gs.sorted_idx[src];
                     if constexpr (NF > 0) {
 #pragma unroll
                         for (int fi = 0; fi < NF; ++fi) fv[gi][fi] = fs[(int64_t)fi * f.fstride + src];
                     }
+                    if constexpr (kPackF) pk[gi].v[3] = fv[gi][0];
                 }
             }
             __syncthreads();  // the previous chunk's pairs are done
@@ -416,9 +431,9 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     if (in) {
                         s_c[ml] = pk[gi];
                         if constexpr (F::kIndex) s_j[ml] = jv[gi];
-                        if constexpr (NF > 0) {
+                        if constexpr (NFL > 0) {
 #pragma unroll
-                            for (int fi = 0; fi < NF; ++fi) s_f[fi][ml] = fv[gi][fi];
+                            for (int fi = FOFF; fi < NF; ++fi) s_f[fi - FOFF][ml] = fv[gi][fi];
                         }
                     }
 #pragma unroll
@@ -430,6 +445,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
 
         // Query batch of up to 64 points starting at sorted position q0.
         auto begin_batch = [&](int q0, int nq, TC (&xq)[D], int64_t& orig_i) {
+            cur_q0 = q0;
             const bool qact = lane < nq;
 #pragma unroll
             for (int d = 0; d < D; ++d) xq[d] = qact ? qcrd[(int64_t)d * Q.cap + q0 + lane] : (TC)0;
@@ -437,11 +453,6 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             if constexpr (!kLight) {
                 __syncthreads();
                 if (qact) {
-                    Pack<TC, D> pk;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) pk.v[d] = xq[d];
-                    if constexpr (D == 3) pk.v[3] = (TC)0;
-                    s_qc[lane] = pk;
                     s_qst[lane] = f.query(orig_i);
                     s_acc[lane] = f.init();
                     if constexpr (F::kFill) s_written[lane] = 0;

@@ -102,8 +102,12 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     constexpr int NFL = NF - FOFF;  // fields kept in their own LDS planes
     __shared__ TR s_f[NFL > 0 ? NFL : 1][(!kLight && NFL > 0) ? CH : 1];
     // pair list (+ one discard slot per lane); a replay copies a cell's records here
-    __shared__ uint64_t s_pairs64[kLight ? 1 : PC_CELL_MAX / 4];
-    uint16_t* const s_pairs = (uint16_t*)s_pairs64;
+    // Entries are candidate slots of the chunk: one byte when a chunk holds at most 256 and no
+    // pair cache is involved (its records are 16-bit).
+    using PairT = typename std::conditional<(PCM == 0 && CH <= 256), uint8_t, uint16_t>::type;
+    constexpr int PAIR_WORDS = (PCM == 0 ? (PAIR_CAP + WAVE) * (int)sizeof(PairT) : 2 * PC_CELL_MAX) / 8;
+    __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];
+    PairT* const s_pairs = (PairT*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
     __shared__ QS s_qst[LQ];
     __shared__ Acc s_acc[LQ];
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         for (int d = 0; d < D; ++d) pk.v[d] = ((const TC*)Q.coords)[(int64_t)d * Q.cap + cur_q0 + q];
         return pk;
     };
-    auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const uint16_t* prp) {
+    auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const PairT* prp) {
         const int nf = i1 - i0;
         s_con[lane] = f.init();
         __syncthreads();
@@ -465,7 +469,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             rbase = pc.head[2 * (int64_t)w];
             rsize = pc.head[2 * (int64_t)w + 1];
         }
-        if (PCM == 2 && rbase >= 0) {
+        if constexpr (PCM == 2) {
+        if (rbase >= 0) {
             // replay: copy the cell's records to LDS (8 bytes per lane and step), then run
             // the recorded flushes batch by batch
             __syncthreads();
@@ -496,6 +501,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 if (lane < nq) f.write(orig_i, s_qst[lane], s_acc[lane]);
             }
             continue;
+        }
         }
 
         for (int q0 = qs, b = 0; q0 < qe; q0 += WAVE, ++b) {
@@ -568,7 +574,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                                 const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
                                 const int below = __builtin_amdgcn_mbcnt_hi(
                                     (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                                s_pairs[acc ? P + below : PAIR_CAP + lane] = (uint16_t)(gi * WAVE + lane);
+                                s_pairs[acc ? P + below : PAIR_CAP + lane] = (PairT)(gi * WAVE + lane);
                                 P += __builtin_popcountll(mask);
                             }
                         }

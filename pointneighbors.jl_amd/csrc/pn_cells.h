@@ -56,13 +56,14 @@ __device__ __forceinline__ T lane_bcast(T v, int i) {
     }
 }
 
-// Candidate groups held in registers per chunk.  Pair bodies: 3 groups (192 candidates) for
-// Float32 by default, which keeps LDS small enough for 4-5 wavefronts per SIMD; the large
-// variant holds a 3D stencil of ~14 points per cell in one chunk.
+// Candidate groups held in registers per chunk.  Counting: 6 (a 3D stencil of ~14 points per
+// cell in one chunk).  Pair bodies: 3 groups (192 candidates) for Float32 by default, which
+// keeps LDS small enough for 6 wavefronts per SIMD; the large variant holds the stencil in
+// one chunk.
 template <class TC, class F>
 struct CellGroups {
-    static constexpr int large = F::kLight ? 8 : (sizeof(TC) == 4 ? 6 : 4);
-    static constexpr int small = F::kLight ? 8 : (sizeof(TC) == 4 ? 3 : 2);
+    static constexpr int large = F::kLight ? 6 : (sizeof(TC) == 4 ? 6 : 4);
+    static constexpr int small = F::kLight ? 6 : (sizeof(TC) == 4 ? 3 : 2);
 };
 
 template <int N>
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     constexpr int FOFF = kPackF ? 1 : 0;
     constexpr int NFL = NF - FOFF;  // fields kept in their own LDS planes
     __shared__ TR s_f[NFL > 0 ? NFL : 1][(!kLight && NFL > 0) ? CH : 1];
-    // pair list (+ one discard slot per lane); a replay copies a cell's records here
+    // pair list; a replay copies a cell's records (up to PC_CELL_MAX entries) here
     // Entries are candidate slots of the chunk: one byte when a chunk holds at most 256 and no
     // pair cache is involved (its records are 16-bit).
     using PairT = typename std::conditional<(PCM == 0 && CH <= 256), uint8_t, uint16_t>::type;
@@ -568,13 +569,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                             for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
 #pragma unroll
                             for (int gi = 0; gi < NG; ++gi) {
-                                // accepted lanes take the next list slots in lane order; the
-                                // others write their own discard slot (no branch)
+                                // accepted lanes take the next list slots in lane order (the
+                                // write runs under their execution mask)
                                 const bool acc = test(xs, gi);
                                 const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
-                                const int below = __builtin_amdgcn_mbcnt_hi(
-                                    (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                                s_pairs[acc ? P + below : PAIR_CAP + lane] = (PairT)(gi * WAVE + lane);
+                                if (acc) {
+                                    const int below = __builtin_amdgcn_mbcnt_hi(
+                                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                                    s_pairs[P + below] = (PairT)(gi * WAVE + lane);
+                                }
                                 P += __builtin_popcountll(mask);
                             }
                         }

@@ -77,10 +77,12 @@ int pn_csr_fill(pn_grid_t grid, int queries, int64_t n_rows, const int32_t* row_
     if (rc) return rc;
     if (n_rows == 0) return PN_OK;
     // persistent grids; the LDS sort only runs when some row is longer than 512 entries
-    const int nblocks = (int)std::min<int64_t>(round_up8(n_rows), 65536);
-    hipLaunchKernelGGL(pn::k_sort_rows, dim3(nblocks), dim3(pn::WAVE), 0, g.stream, row_ptr, n_rows, col_idx,
-                       nblocks);
-    PN_CHECK_LAUNCH();
+    {
+        const int nblocks = (int)std::min<int64_t>(round_up8(n_rows), 65536);
+        hipLaunchKernelGGL(pn::k_sort_rows, dim3(nblocks), dim3(pn::WAVE), 0, g.stream, row_ptr, n_rows, col_idx,
+                           nblocks);
+        PN_CHECK_LAUNCH();
+    }
     if (!g.csr_host_status || g.csr_host_status->max_count > pn::SORT_REGS_MAX) {
         const int nb_lds = (int)std::min<int64_t>(n_rows, 4096);
         hipLaunchKernelGGL(pn::k_sort_rows_lds, dim3(nb_lds), dim3(pn::WAVE), 0, g.stream, row_ptr, n_rows, col_idx,

@@ -110,7 +110,10 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];
     PairT* const s_pairs = (PairT*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
-    __shared__ QS s_qst[LQ];
+    // Per-query state (f.query): kept in LDS when it is one word, else reloaded from global
+    // memory where a flush needs it (larger states would cost occupancy).
+    constexpr bool kQstLds = sizeof(QS) <= 4;
+    __shared__ QS s_qst[kQstLds ? LQ : 1];
     __shared__ Acc s_acc[LQ];
     __shared__ Acc s_con[LQ];
     __shared__ int s_written[F::kFill ? WAVE : 1];
@@ -172,6 +175,11 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         for (int d = 0; d < D; ++d) pk.v[d] = ((const TC*)Q.coords)[(int64_t)d * Q.cap + cur_q0 + q];
         return pk;
     };
+    // ... and its per-query state.
+    auto qstate = [&](int q) -> QS {
+        if constexpr (kQstLds) return s_qst[q];
+        else return f.query((int64_t)Q.sorted_idx[cur_q0 + q]);
+    };
     auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const PairT* prp) {
         const int nf = i1 - i0;
         s_con[lane] = f.init();
@@ -198,14 +206,18 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         if constexpr (F::kFill) {
             // CSR fill: pair k goes to lane k % 64, so a store instruction writes consecutive
             // slots of one or two rows; each lane's query cursor only moves forward.
-            int fq = 0, nxt = qsp[1];
+            int fq = -1, nxt = 0, row0 = 0;  // row0: slot of pair qsp[fq] of query fq
+            QS qst{};
             for (int k = lane; k < P; k += WAVE) {
-                while (k >= nxt) {
-                    ++fq;
-                    nxt = qsp[fq + 1];
+                if (k >= nxt) {
+                    do {
+                        ++fq;
+                        nxt = qsp[fq + 1];
+                    } while (k >= nxt);
+                    qst = qstate(i0 + fq);
+                    row0 = qst.rb + s_written[i0 + fq] - (int)qsp[fq];
                 }
-                const QS qst = s_qst[i0 + fq];
-                const int slot = qst.rb + s_written[i0 + fq] + (k - (int)qsp[fq]);
+                const int slot = row0 + k;
                 if (slot < qst.re) f.col[slot] = s_j[prp[k]] + f.base;
             }
             __syncthreads();
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             int fq = lo;
             int next = qsp[fq + 1];
             Pack<TC, D> qp = qpack(i0 + fq);
-            QS qst = s_qst[i0 + fq];
+            QS qst = qstate(i0 + fq);
             Acc a = f.init();
             auto finish = [&]() {
                 const bool starts = qsp[fq] >= kb;
@@ -249,7 +261,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                         next = qsp[fq + 1];
                     } while (k >= next);
                     qp = qpack(i0 + fq);
-                    qst = s_qst[i0 + fq];
+                    qst = qstate(i0 + fq);
                     a = f.init();
                 }
                 const int m = prp[k];
@@ -458,7 +470,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             if constexpr (!kLight) {
                 __syncthreads();
                 if (qact) {
-                    s_qst[lane] = f.query(orig_i);
+                    if constexpr (kQstLds) s_qst[lane] = f.query(orig_i);
                     s_acc[lane] = f.init();
                     if constexpr (F::kFill) s_written[lane] = 0;
                 }
@@ -499,7 +511,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     pos += 5 + nf + P;
                 }
                 __syncthreads();
-                if (lane < nq) f.write(orig_i, s_qst[lane], s_acc[lane]);
+                if (lane < nq) f.write(orig_i, qstate(lane), s_acc[lane]);
             }
             continue;
         }
@@ -590,8 +602,9 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 if (qact) f.write(orig_i, QS{}, mycount);
             } else {
                 __syncthreads();
-                if (qact) f.write(orig_i, s_qst[lane], s_acc[lane]);
+                if (qact) f.write(orig_i, qstate(lane), s_acc[lane]);
             }
+
         }
         if constexpr (F::kCache && PCM == 1) {
             {

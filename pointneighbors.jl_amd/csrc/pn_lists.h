@@ -106,6 +106,29 @@ __device__ __forceinline__ void sort_row_regs(int32_t* __restrict__ col, int b, 
 // (up to SORT_CAP).
 constexpr int SORT_REGS_MAX = 8 * WAVE;
 
+// Sorts rows [b0, b0 + n0) and [b1, b1 + n1) of col (wave-uniform arguments): together when
+// both fit two registers per lane, else one at a time; rows longer than SORT_REGS_MAX are left
+// to k_sort_rows_lds.
+__device__ __forceinline__ void sort_row_pair(int32_t* __restrict__ col, int b0, int n0, int b1, int n1, int lane) {
+    const int m = max(n0, n1);
+    if (m <= 2 * WAVE) {
+        const int bb[2] = {b0, b1}, nn[2] = {n0 > 1 ? n0 : 0, n1 > 1 ? n1 : 0};
+        if (m <= WAVE) sort_rows_regs<1, 2>(col, bb, nn, lane);
+        else sort_rows_regs<2, 2>(col, bb, nn, lane);
+        return;
+    }
+    const int bs[2] = {b0, b1}, ns[2] = {n0, n1};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int n = ns[r], b = bs[r];
+        if (n <= 1 || n > SORT_REGS_MAX) continue;
+        if (n <= WAVE) sort_row_regs<1>(col, b, n, lane);
+        else if (n <= 2 * WAVE) sort_row_regs<2>(col, b, n, lane);
+        else if (n <= 4 * WAVE) sort_row_regs<4>(col, b, n, lane);
+        else sort_row_regs<8>(col, b, n, lane);
+    }
+}
+
 static __global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __restrict__ row_ptr, int64_t n_rows,
                                                     int32_t* __restrict__ col, int nblocks) {
     const int lane = threadIdx.x;
@@ -116,25 +139,7 @@ static __global__ __launch_bounds__(WAVE) void k_sort_rows(const int32_t* __rest
         const bool two = row + 1 < r_end;
         const int b0 = row_ptr[row], e0 = row_ptr[row + 1];
         const int e1 = two ? row_ptr[row + 2] : e0;
-        const int n0 = e0 - b0, n1 = e1 - e0;
-        const int m = max(n0, n1);
-        if (m <= 2 * WAVE) {
-            // both rows fit two registers per lane: sort them together
-            const int bb[2] = {b0, e0}, nn[2] = {n0 > 1 ? n0 : 0, n1 > 1 ? n1 : 0};
-            if (m <= WAVE) sort_rows_regs<1, 2>(col, bb, nn, lane);
-            else sort_rows_regs<2, 2>(col, bb, nn, lane);
-            continue;
-        }
-        const int bs[2] = {b0, e0}, ns[2] = {n0, n1};
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int n = ns[r], b = bs[r];
-            if (n <= 1 || n > SORT_REGS_MAX) continue;
-            if (n <= WAVE) sort_row_regs<1>(col, b, n, lane);
-            else if (n <= 2 * WAVE) sort_row_regs<2>(col, b, n, lane);
-            else if (n <= 4 * WAVE) sort_row_regs<4>(col, b, n, lane);
-            else sort_row_regs<8>(col, b, n, lane);
-        }
+        sort_row_pair(col, b0, e0 - b0, e0, e1 - e0, lane);
     }
 }
 

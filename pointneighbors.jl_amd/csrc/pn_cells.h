@@ -110,9 +110,10 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];
     PairT* const s_pairs = (PairT*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
-    // Per-query state (f.query): kept in LDS when it is one word, else reloaded from global
-    // memory where a flush needs it (larger states would cost occupancy).
-    constexpr bool kQstLds = sizeof(QS) <= 4;
+    // Per-query state (f.query): kept in LDS when it is one word or the CSR fill's row bounds
+    // (read at almost every pair), else reloaded from global memory where a flush needs it
+    // (larger states would cost occupancy).
+    constexpr bool kQstLds = sizeof(QS) <= 4 || F::kFill;
     __shared__ QS s_qst[kQstLds ? LQ : 1];
     __shared__ Acc s_acc[LQ];
     __shared__ Acc s_con[LQ];

@@ -195,6 +195,8 @@ int pn_list_sum_neighbor_indices(const int32_t* row_ptr, const int32_t* col_idx,
  *   pn_pair_energy            e_i = 1/2 sum_{j != i} eps (1 - r/rc)^4 (1 + 4 r/rc), r < rc
  *   *_vjp                     vector-Jacobian products for upstream gradients of the outputs
  *   pn_tait_eos               p = B ((rho/rho0)^gamma - 1), B = rho0 c^2 / gamma
+ *   pn_tait_pressure_term     P = p / rho^2 with the same p: the per-point term of the pressure
+ *                             acceleration, in one pass
  * ---------------------------------------------------------------------------------- */
 enum pn_kernel { PN_KERNEL_CUBIC_SPLINE = 0, PN_KERNEL_WENDLAND_C2 = 1 };
 
@@ -213,6 +215,8 @@ int pn_pair_energy_vjp(pn_grid_t grid, double eps, double cutoff, const void* e_
                        void* x_bar, int accumulate);
 int pn_tait_eos(const void* rho, void* p, int64_t n, int dtype, double rho0, double c,
                 double gamma, void* stream);
+int pn_tait_pressure_term(const void* rho, void* P, int64_t n, int dtype, double rho0, double c,
+                          double gamma, void* stream);
 
 /* ------------------------------------------------------------------------------------
  * Batches of independent clouds.  No reference counterpart: the nearest concept is one

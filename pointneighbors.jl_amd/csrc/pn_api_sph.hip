@@ -65,20 +65,32 @@ int pn_pair_energy(pn_grid_t grid, int queries, double eps, double cutoff, void*
     });
 }
 
-int pn_tait_eos(const void* rho, void* p, int64_t n, int dtype, double rho0, double c, double gamma, void* stream) {
-    if (!rho || !p) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+namespace {
+int tait_eos(const void* rho, void* out, int64_t n, int dtype, double rho0, double c, double gamma, int over_rho2,
+             void* stream) {
+    if (!rho || !out) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     if (n <= 0) return PN_OK;
     if (dtype == PN_FLOAT32) {
         hipLaunchKernelGGL(pn::k_tait_eos<float>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                           (const float*)rho, (float*)p, n, (float)rho0, (float)c, (float)gamma);
+                           (const float*)rho, (float*)out, n, (float)rho0, (float)c, (float)gamma, over_rho2);
     } else if (dtype == PN_FLOAT64) {
         hipLaunchKernelGGL(pn::k_tait_eos<double>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                           (const double*)rho, (double*)p, n, rho0, c, gamma);
+                           (const double*)rho, (double*)out, n, rho0, c, gamma, over_rho2);
     } else {
         return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
     }
     PN_CHECK_LAUNCH();
     return PN_OK;
+}
+}  // namespace
+
+int pn_tait_eos(const void* rho, void* p, int64_t n, int dtype, double rho0, double c, double gamma, void* stream) {
+    return tait_eos(rho, p, n, dtype, rho0, c, gamma, 0, stream);
+}
+
+int pn_tait_pressure_term(const void* rho, void* P, int64_t n, int dtype, double rho0, double c, double gamma,
+                          void* stream) {
+    return tait_eos(rho, P, n, dtype, rho0, c, gamma, 1, stream);
 }
 
 }  // extern "C"

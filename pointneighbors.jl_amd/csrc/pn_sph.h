@@ -350,13 +350,16 @@ struct PressureVjpF : FunctorBase {
 };
 
 // Tait equation of state, pointwise: p = B ((rho / rho0)^gamma - 1), B = rho0 c^2 / gamma.
+// over_rho2 != 0 writes the pressure-acceleration term P = p / rho^2 instead of p.
 template <class TR>
 __global__ __launch_bounds__(256) void k_tait_eos(const TR* __restrict__ rho, TR* __restrict__ p, int64_t n, TR rho0,
-                                                  TR c, TR gamma) {
+                                                  TR c, TR gamma, int over_rho2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const TR B = rho0 * c * c / gamma;
-    p[i] = B * (pow(rho[i] / rho0, gamma) - (TR)1);
+    const TR r = rho[i];
+    const TR pi = B * (pow(r / rho0, gamma) - (TR)1);
+    p[i] = over_rho2 ? pi / (r * r) : pi;
 }
 
 }  // namespace pn

@@ -81,6 +81,7 @@ SIGNATURES = {
     "pn_pair_energy": (_c_int, [_vp, _c_int, _c_dbl, _c_dbl, _vp, _c_int]),
     "pn_pair_energy_vjp": (_c_int, [_vp, _c_dbl, _c_dbl, _vp, _vp, _c_int]),
     "pn_tait_eos": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl, _vp]),
+    "pn_tait_pressure_term": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl, _vp]),
     "pn_batch_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_i64, _p_dbl, _p_i64, _c_dbl, _c_i32,
                                  _p_i64, ctypes.POINTER(_vp)]),
     "pn_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_i64]),

@@ -154,6 +154,18 @@ def tait_eos(rho, rho0, c, gamma=7.0):
     return p
 
 
+def tait_pressure_term(rho, rho0, c, gamma=7.0, out=None):
+    """P = p / rho^2 with p = tait_eos(rho, ...): the per-point term that
+    SPHPressureAcceleration takes, in one HIP kernel."""
+    import torch
+    P = torch.empty_like(rho) if out is None else out
+    code = 0 if rho.dtype == torch.float32 else 1
+    check(load().pn_tait_pressure_term(_vp(rho), _vp(P), rho.numel(), code, float(rho0), float(c), float(gamma),
+                                       ctypes.c_void_p(_device.current_stream_handle(rho.device))),
+          "tait_pressure_term")
+    return P
+
+
 def _vec_like(x):
     """A zeroed D x N tensor with the coordinate layout (strides (1, D))."""
     import torch

@@ -65,6 +65,12 @@ def test_eos_kernel(orc):
     rho = np.linspace(990.0, 1010.0, 1000)
     p = sph.tait_eos(tdev(rho, np.float64), 1000.0, 10.0, 7.0).cpu().numpy()
     assert np.allclose(p, orc.tait_eos(rho, 1000.0, 10.0, 7.0), rtol=1e-13, atol=1e-9)
+    # the fused pressure term P = p / rho^2
+    want = orc.tait_eos(rho, 1000.0, 10.0, 7.0) / rho**2
+    P = sph.tait_pressure_term(tdev(rho, np.float64), 1000.0, 10.0, 7.0).cpu().numpy()
+    assert np.allclose(P, want, rtol=1e-13, atol=1e-15)
+    P32 = sph.tait_pressure_term(tdev(rho, np.float32), 1000.0, 10.0, 7.0).cpu().numpy()
+    assert np.allclose(P32, want, rtol=0, atol=2e-4 * np.abs(want).max())
 
 
 def test_pair_energy(orc):

@@ -61,6 +61,31 @@ def test_density_and_pressure(orc, dt, tol, kernel):
     assert (err <= tol * np.maximum(scale, 1e-300)).all(), (err / scale).max()
 
 
+@pytest.mark.parametrize("dt,tol", [(np.float32, 1e-5), (np.float64, 1e-12)])
+@pytest.mark.parametrize("kernel", ["cubic_spline", "wendland_c2"])
+def test_density_and_pressure_2d(orc, dt, tol, kernel):
+    """2D: candidate records without a padding slot, 2D kernel normalisations."""
+    sp = 0.02
+    pts = synthetic.jittered_lattice((70, 61), sp, 0.15 * sp, dtype=dt, seed=4)
+    h = dt(1.3) * dt(sp)
+    r = dt(2) * h
+    nhs, x, lists = setup(orc, pts, r, dt, dt)
+    n = len(pts)
+    m = np.full(n, 1000.0 * sp**2, dtype=dt)
+    rho = torch.zeros(n, dtype=x.dtype, device="cuda")
+    pn.foreach_point_neighbor(sph.SPHDensity(rho, tdev(m, dt), float(h), kernel), x, x, nhs)
+    orho, scale = orc.sph_density(pts, lists, m, float(h), kernel)
+    err = np.abs(rho.cpu().numpy().astype(np.float64) - orho)
+    assert (err <= tol * scale).all(), err.max()
+    P = (orc.tait_eos(orho, 1000.0, 10.0) / orho**2).astype(dt)
+    dv = sph._vec_like(x).zero_()
+    pn.foreach_point_neighbor(sph.SPHPressureAcceleration(dv, tdev(m, dt), tdev(P, dt), float(h), kernel),
+                              x, x, nhs)
+    odv, scale = orc.sph_pressure_accel(pts, lists, m, P, float(h), kernel)
+    err = np.abs(dv.t().cpu().numpy().astype(np.float64) - odv).max(axis=1)
+    assert (err <= tol * np.maximum(scale, 1e-300)).all(), (err / scale).max()
+
+
 def test_eos_kernel(orc):
     rho = np.linspace(990.0, 1010.0, 1000)
     p = sph.tait_eos(tdev(rho, np.float64), 1000.0, 10.0, 7.0).cpu().numpy()
@@ -113,37 +138,41 @@ def losses(which, x, nhs, m, kw):
     return (a * a).sum()
 
 
+@pytest.mark.parametrize("shape,dt,tol", [((14, 15, 13), np.float64, 1e-9), ((40, 37), np.float64, 1e-9),
+                                          ((14, 15, 13), np.float32, 5e-3)])
 @pytest.mark.parametrize("which", [0, 1, 2])
-def test_gradients_vs_dual_and_fd(orc, which):
-    pts = synthetic.jittered_lattice((14, 15, 13), 1.0, 0.2, dtype=np.float64, seed=6)
+def test_gradients_vs_dual_and_fd(orc, which, shape, dt, tol):
+    pts = synthetic.jittered_lattice(shape, 1.0, 0.2, dtype=dt, seed=6)
+    nd = len(shape)
     cutoff = 2.5
     if which == 2:
         kw = dict(kernel="cubic_spline", h=1.2, rc=2.4, eps=1.0, rho0=1.0, c=10.0, gamma=7.0)
         cutoff = 2.4
     else:
         kw = dict(kernel="wendland_c2", h=cutoff / 2, rc=cutoff, eps=1.3, rho0=1.0, c=10.0, gamma=7.0)
-    nhs, x, lists = setup(orc, pts, cutoff, np.float64, np.float64)
-    m = torch.full((len(pts),), lattice_mass("wendland_c2", cutoff / 2, cutoff), dtype=torch.float64,
-                   device="cuda")
+    nhs, x, lists = setup(orc, pts, dt(cutoff), dt, dt)
+    m = torch.full((len(pts),), lattice_mass("wendland_c2", cutoff / 2, cutoff), dtype=x.dtype, device="cuda")
     xg = x.detach().clone().requires_grad_(True)
     # the differentiable functions need the grid built from xg's own storage
     pn.initialize(nhs, xg.detach(), xg.detach())
     L = losses(which, xg, nhs, m, kw)
     L.backward()
-    g = xg.grad.t().cpu().numpy()
-    assert np.abs(g.sum(axis=0)).max() <= 1e-10 * np.abs(g).sum(), "translation invariance"
+    g = xg.grad.t().cpu().numpy().astype(np.float64)
+    assert np.abs(g.sum(axis=0)).max() <= (1e-10 if dt == np.float64 else 1e-4) * np.abs(g).sum(), \
+        "translation invariance"
     n = len(pts)
     idx = (synthetic.uniform(100 + which, 12) * n).astype(int)
     ref = []
     got = []
     for a, k in enumerate(idx):
-        dim = a % 3
-        _, d = orc.loss_dual(which, pts, lists, k, dim, mass=m.cpu().numpy(), kernel=kw["kernel"], h=kw["h"],
-                             eps=kw["eps"], rc=kw["rc"], rho0=kw["rho0"], c=kw["c"], gamma=kw["gamma"])
+        dim = a % nd
+        _, d = orc.loss_dual(which, pts.astype(np.float64), lists, k, dim, mass=m.cpu().numpy().astype(np.float64),
+                             kernel=kw["kernel"], h=kw["h"], eps=kw["eps"], rc=kw["rc"], rho0=kw["rho0"],
+                             c=kw["c"], gamma=kw["gamma"])
         ref.append(d)
         got.append(g[k, dim])
     ref, got = np.array(ref), np.array(got)
-    assert rel_err(got, ref).max() < 1e-9, (got, ref)
+    assert rel_err(got, ref).max() < tol, (got, ref)
 
 
 def test_fd_density_loss_large(orc):

@@ -130,10 +130,11 @@ def main():
 
     pn.initialize(nhs, x, x)
     tuned = None
+    tune_table = None
     if args.autotune:
         from pointneighbors import autotune
         key = autotune.signature(nhs, x, args.workload)
-        tuned, _ = autotune.autotune(nhs, step, reps=5, warmup=2, cache=args.tuning_cache, key=key)
+        tuned, tune_table = autotune.autotune(nhs, step, reps=5, warmup=2, cache=args.tuning_cache, key=key)
     if args.tune:
         tuned = dict(tuned or {})
         for kv in args.tune:
@@ -236,6 +237,7 @@ def main():
                      "algorithmic_bytes_per_launch": algo},
         "cpu_baseline": cpu,
         "tuning": tuned,
+        "autotune_table": tune_table,
     }
     print(json.dumps(line))
 

@@ -72,6 +72,8 @@ def lib():
                                              ctypes.c_float, ctypes.c_float, vp, vp, vp, vp]
         L.orc_sph_density.restype = None
         L.orc_sph_density.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_tlsph_grad.restype = None
+        L.orc_tlsph_grad.argtypes = [ci, ci, dbl, vp, vp, i64, vp, vp, vp, vp, vp]
         L.orc_sph_pressure_accel.restype = None
         L.orc_sph_pressure_accel.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp, vp]
         L.orc_pair_energy.restype = None
@@ -317,6 +319,20 @@ def sph_density(x_nd, lists, mass, h, kernel="cubic_spline"):
     lib().orc_sph_density(x.shape[1], KERNELS[kernel], float(h), _ptr(x), x.shape[0], _ptr(rp), _ptr(cols),
                           _ptr(m), _ptr(rho), _ptr(scale))
     return rho, scale
+
+
+def tlsph_grad(X_nd, x_nd, lists, volume, h, kernel="wendland_c2", L=None):
+    """TLSPH deformation gradient (oracle restatement): (N, D, D) with out[a] = G_a L_a."""
+    X = np.ascontiguousarray(X_nd, dtype=np.float64)
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    v = np.ascontiguousarray(volume, dtype=np.float64)
+    n, nd = X.shape
+    out = np.zeros((n, nd, nd))
+    Lp = None if L is None else np.ascontiguousarray(L, dtype=np.float64)
+    lib().orc_tlsph_grad(nd, KERNELS[kernel], float(h), _ptr(X), _ptr(x), n, _ptr(rp), _ptr(cols), _ptr(v),
+                         None if Lp is None else _ptr(Lp), _ptr(out))
+    return out
 
 
 def sph_pressure_accel(x_nd, lists, mass, P, h, kernel="cubic_spline"):

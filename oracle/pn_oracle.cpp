@@ -821,6 +821,46 @@ void orc_pair_energy(int nd, double eps, double rc, const double* x, int64_t n, 
     p.energy(eps, rc, e);
 }
 
+// TLSPH deformation gradient over CSR lists of the initial coordinates X (TrixiParticles 0.5
+// calc_deformation_grad!, run by benchmarks/smoothed_particle_hydrodynamics.jl:127-146):
+//     G_a = sum_{b in list(a), |X_a - X_b| >= sqrt(eps(Float64))} V_b (x_b - x_a) (W'(r) X_ab / r)^T,
+// out_a = G_a L_a (L != nullptr, row-major D x D per point here) or G_a.  TrixiParticles is not
+// vendored; the sum is restated from its published algorithm and pinned by closed forms (an
+// undeformed body gives the symmetric correction matrix M; with L = M^-1 an affine map x = A X + c
+// gives A exactly).  Row-major D x D output per point.
+void orc_tlsph_grad(int nd, int kernel, double h, const double* X, const double* x, int64_t n, const int64_t* rp,
+                    const int32_t* cols, const double* vol, const double* L, double* out) {
+    const Kern<double> k{kernel, nd, h};
+    const double tiny = 1.4901161193847656e-8;
+    for (int64_t a = 0; a < n; ++a) {
+        double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int64_t q = rp[a]; q < rp[a + 1]; ++q) {
+            const int64_t b = cols[q];
+            double Xab[3] = {0, 0, 0}, d2 = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                Xab[d] = X[a * nd + d] - X[b * nd + d];
+                d2 += Xab[d] * Xab[d];
+            }
+            const double r = std::sqrt(d2);
+            if (r < tiny) continue;
+            const double s = vol[b] * k.dw(r) / r;
+            for (int i = 0; i < nd; ++i)
+                for (int j = 0; j < nd; ++j) g[i][j] += s * (x[b * nd + i] - x[a * nd + i]) * Xab[j];
+        }
+        double* o = out + a * nd * nd;
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                double v = 0.0;
+                if (L) {
+                    for (int m = 0; m < nd; ++m) v += g[i][m] * L[a * nd * nd + m * nd + j];
+                } else {
+                    v = g[i][j];
+                }
+                o[i * nd + j] = v;
+            }
+    }
+}
+
 // Scalar losses in dual numbers, derivative along coordinate (k, dim) (forward mode):
 //   which 0: E = sum_i e_i (Wendland pair energy, eps, rc = cutoff)
 //   which 1: L = sum_i (rho_i - rho0)^2, rho with `kernel`, mass m

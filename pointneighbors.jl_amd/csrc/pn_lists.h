@@ -295,4 +295,107 @@ __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict_
     if (sub == 0) rho[i] = accumulate ? rho[i] + s : s;
 }
 
+// TLSPH deformation gradient over stored lists (TrixiParticles 0.5 calc_deformation_grad!,
+// run by benchmarks/smoothed_particle_hydrodynamics.jl:127-146 through
+// foreach_point_neighbor over the precomputed lists of the initial coordinates):
+//     G_a = sum_{b in list(a), |X_ab| >= sqrt(eps)} V_b (x_b - x_a) (grad_a W(X_ab))^T,
+//     grad_a W(X_ab) = W'(|X_ab|) X_ab / |X_ab|,   X_ab = X_a - X_b,
+// out_a = G_a L_a (L given) or G_a.  With x = X this is the (symmetric) kernel-gradient
+// correction matrix M_a, and L = M^-1 makes G reproduce affine maps exactly.  Sixteen lanes
+// per row as in k_list_density; each lane keeps its D x D partial sum, the butterfly over the
+// 16 lanes has a fixed order.  X, x: D x N; out and L: D x D per point, column-major.
+template <int D, class T, class K>
+__global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restrict__ row_ptr,
+                                                         const int32_t* __restrict__ col, int stored_base,
+                                                         int64_t sel_first, const int32_t* __restrict__ sel_list,
+                                                         int64_t n_sel, const T* __restrict__ X, const T* __restrict__ x,
+                                                         const T* __restrict__ vol, K kern, const T* __restrict__ L,
+                                                         T* __restrict__ out, int nblocks) {
+    const int sub = threadIdx.x & (LIST_LANES - 1);
+    const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
+    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    const int64_t i = sel_point(a, sel_first, sel_list);
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    T Xi[D], xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        Xi[d] = X[(int64_t)D * i + d];
+        xi[d] = x[(int64_t)D * i + d];
+    }
+    const T tiny = (T)1.4901161193847656e-8;  // sqrt(eps(Float64)): coincident points add nothing
+    T g[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) g[r][c] = (T)0;
+    }
+    for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * LIST_UNROLL) {
+        int64_t j[LIST_UNROLL];
+#pragma unroll
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            const int k = k0 + LIST_LANES * u;
+            j[u] = k < e ? (int64_t)col[k] - stored_base : -1;
+        }
+        T Xj[LIST_UNROLL][D], xj[LIST_UNROLL][D], vj[LIST_UNROLL];
+#pragma unroll
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            const int64_t jj = j[u] >= 0 ? j[u] : 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                Xj[u][d] = X[(int64_t)D * jj + d];
+                xj[u][d] = x[(int64_t)D * jj + d];
+            }
+            vj[u] = vol[jj];
+        }
+#pragma unroll
+        for (int u = 0; u < LIST_UNROLL; ++u) {
+            T Xab[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) Xab[d] = Xi[d] - Xj[u][d];
+            T d2 = Xab[0] * Xab[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
+            const T dist = sqrt(d2);
+            const bool use = j[u] >= 0 && dist >= tiny;
+            // V_b W'(r) / r, zero for pairs that do not contribute
+            const T s = use ? vj[u] * kern.dw(dist) / dist : (T)0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                const T dx = xj[u][r] - xi[r];
+#pragma unroll
+                for (int c = 0; c < D; ++c) g[r][c] = fma(s * dx, Xab[c], g[r][c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = LIST_LANES / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[r][c] += __shfl_xor(g[r][c], o, LIST_LANES);
+        }
+    }
+    if (sub != 0) return;
+    T* o = out + (int64_t)D * D * i;
+    if (L) {
+        const T* l = L + (int64_t)D * D * i;  // column-major: l[r + D c] = L[r][c]
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                T v = g[r][0] * l[0 + D * c];
+#pragma unroll
+                for (int k = 1; k < D; ++k) v = fma(g[r][k], l[k + D * c], v);
+                o[r + D * c] = v;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+#pragma unroll
+            for (int r = 0; r < D; ++r) o[r + D * c] = g[r][c];
+        }
+    }
+}
+
 }  // namespace pn

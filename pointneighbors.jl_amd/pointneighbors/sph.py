@@ -154,6 +154,55 @@ def tait_eos(rho, rho0, c, gamma=7.0):
     return p
 
 
+def tlsph_deformation_gradient(nhs, X, x, volume, smoothing_length, kernel="wendland_c2", correction=None,
+                               points=None):
+    """TLSPH deformation gradient over the stored lists of `nhs`, a PrecomputedNeighborhoodSearch
+    initialized with the initial coordinates X (TrixiParticles 0.5 calc_deformation_grad!, as run by
+    benchmarks/smoothed_particle_hydrodynamics.jl:127-146):
+
+        F_a = sum_b V_b (x_b - x_a) (grad_a W(X_a - X_b))^T L_a,   |X_a - X_b| >= sqrt(eps)
+
+    X, x: D x N coordinates; volume: V = m / rho0 per point; correction: L as an (N, D, D)
+    tensor (None: identity).  Returns F as an (N, D, D) tensor (F[a] is F_a), one HIP kernel."""
+    import torch
+    from .util import dtype_code
+    nl = getattr(nhs, "neighbor_lists", None)
+    if nl is None:
+        raise ArgumentError("tlsph_deformation_gradient needs a PrecomputedNeighborhoodSearch")
+    if nhs.periodic_box is not None:
+        raise ArgumentError("tlsph_deformation_gradient does not support a PeriodicBox")
+    D = nhs.ndims
+    X = _device.as_coordinates(X, D, "initial coordinates")
+    x = _device.as_coordinates(x, D, "current coordinates")
+    if X.dtype != x.dtype or X.shape != x.shape:
+        raise ArgumentError("X and x must have the same dtype and shape")
+    n = X.shape[1]
+    _field(volume, n, X.dtype, "volume")
+    out = torch.empty((n, D, D), dtype=X.dtype, device=X.device)  # out[a] holds F_a column-major
+    L = None
+    if correction is not None:
+        if tuple(correction.shape) != (n, D, D) or correction.dtype != X.dtype:
+            raise ArgumentError(f"correction must be an ({n}, {D}, {D}) tensor of {X.dtype}")
+        L = correction.transpose(1, 2).contiguous()
+    first, count, ptr, keep = _device.selection(points, n, X.device)
+    check(load().pn_list_tlsph_deformation_grad(
+        _vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, D, dtype_code(_device.np_dtype(X)), _kernel_code(kernel),
+        float(smoothing_length), _vp(X), _vp(x), _vp(volume), _vp(L) if L is not None else None, first, count,
+        ctypes.c_void_p(ptr) if ptr else None, _vp(out),
+        ctypes.c_void_p(_device.current_stream_handle(X.device))), "tlsph_deformation_gradient")
+    del keep
+    return out.transpose(1, 2)
+
+
+def tlsph_correction_matrix(nhs, X, volume, smoothing_length, kernel="wendland_c2"):
+    """Kernel-gradient correction L_a = M_a^-1, M_a = sum_b V_b (X_b - X_a) (grad_a W(X_ab))^T
+    (the deformation gradient of the undeformed body).  M is a HIP list traversal; the D x D
+    inverses are a batched torch call."""
+    import torch
+    M = tlsph_deformation_gradient(nhs, X, X, volume, smoothing_length, kernel)
+    return torch.linalg.inv(M)
+
+
 def tait_pressure_term(rho, rho0, c, gamma=7.0, out=None):
     """P = p / rho^2 with p = tait_eos(rho, ...): the per-point term that
     SPHPressureAcceleration takes, in one HIP kernel."""

@@ -120,3 +120,57 @@ def test_energy_gradient_translation_invariance(orc):
         for dim in range(3):
             g[dim] += orc.loss_dual(0, pts, lists, k, dim, eps=1.0, rc=2.5)[1]
     assert np.abs(g).max() < 1e-12 * len(pts)
+
+
+# ---- TLSPH deformation gradient (TrixiParticles 0.5 calc_deformation_grad!) ----------------
+
+@pytest.mark.parametrize("kernel", ["wendland_c2", "cubic_spline"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_tlsph_grad_closed_forms(orc, kernel, nd):
+    """The correction matrix M (x = X) is symmetric; with L = M^-1 an affine map x = A X + c
+    gives F = A at every point (boundary points included), a rotation gives R, and the
+    undeformed body gives I.  A transposed outer product or a wrong sign fails these."""
+    shape = (9, 8, 7) if nd == 3 else (14, 13)
+    X = synthetic.jittered_lattice(shape, 1.0, 0.15, dtype=np.float64, seed=21)
+    h = 1.3
+    lists = neighbor_lists(orc, X, 2 * h)
+    V = 1.0 + 0.1 * synthetic.uniform(5, len(X))
+    M = orc.tlsph_grad(X, X, lists, V, h, kernel)
+    assert np.allclose(M, M.transpose(0, 2, 1), rtol=0, atol=1e-12 * np.abs(M).max())
+    L = np.linalg.inv(M)
+    rng = np.random.default_rng(3)
+    A = np.eye(nd) + 0.3 * rng.standard_normal((nd, nd))
+    c = rng.standard_normal(nd)
+    F = orc.tlsph_grad(X, X @ A.T + c, lists, V, h, kernel, L=L)
+    assert np.allclose(F, A, rtol=0, atol=1e-10)
+    th = 0.7
+    R = np.eye(nd)
+    R[:2, :2] = [[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]]
+    F = orc.tlsph_grad(X, X @ R.T, lists, V, h, kernel, L=L)
+    assert np.allclose(F, R, rtol=0, atol=1e-10)
+    assert np.allclose(orc.tlsph_grad(X, X, lists, V, h, kernel, L=L), np.eye(nd), rtol=0, atol=1e-10)
+    # uncorrected, in the interior of a regular lattice M is close to the identity
+    lat = synthetic.jittered_lattice(shape, 1.0, 0.0, dtype=np.float64)
+    Ml = orc.tlsph_grad(lat, lat, neighbor_lists(orc, lat, 2 * h), np.ones(len(lat)), h, kernel)
+    centre = int(np.argmin(np.linalg.norm(lat - lat.mean(axis=0), axis=1)))
+    assert np.allclose(Ml[centre], np.eye(nd), atol=0.05)
+
+
+def test_tlsph_grad_translation_and_locality(orc):
+    """F depends on x only through differences (a translation changes nothing), and a point
+    displaced outside every neighbour's support changes only rows that list it."""
+    X = synthetic.jittered_lattice((8, 8, 8), 1.0, 0.2, dtype=np.float64, seed=4)
+    h = 1.2
+    lists = neighbor_lists(orc, X, 2 * h)
+    V = np.ones(len(X))
+    x = X + 0.05 * np.sin(X)
+    F = orc.tlsph_grad(X, x, lists, V, h)
+    assert np.allclose(orc.tlsph_grad(X, x + np.array([3.0, -1.0, 2.5]), lists, V, h), F, rtol=0, atol=1e-12)
+    rp, cols = lists
+    k = 100
+    x2 = x.copy()
+    x2[k] += 0.3
+    F2 = orc.tlsph_grad(X, x2, lists, V, h)
+    rows = {a for a in range(len(X)) if k in cols[rp[a]:rp[a + 1]]}
+    changed = {a for a in range(len(X)) if not np.allclose(F2[a], F[a], rtol=0, atol=1e-14)}
+    assert changed <= rows and k in changed

@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr"], default="wcsph")
+    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr", "tlsph"], default="wcsph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -82,6 +82,9 @@ def main():
         return
     if args.workload == "csr":
         print(json.dumps(bench_csr(args, dev)))
+        return
+    if args.workload == "tlsph":
+        print(json.dumps(bench_tlsph(args, dev)))
         return
     pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
     n = pts.shape[0]
@@ -462,6 +465,39 @@ def bench_csr(args, dev):
                    "n_points": n},
         "nnz": nnz, "max_row": int(lens.max().item()), "interior_row": int(lens[64 + 128 * 64 + 128 * 128 * 64].item()),
         "build_ms": build_ms, "list_density_ms": sum_ms, "list_pairs_per_s": nnz / (sum_ms * 1e-3),
+    }
+
+
+def bench_tlsph(args, dev):
+    """TLSPH deformation gradient (TrixiParticles calc_deformation_grad!, the reference's
+    benchmark_tlsph_deformation_grad) on a 128^3 Float32 lattice, spacing 1, search radius 3
+    (h = 1.5): precomputed lists of the initial coordinates, the correction matrix once, then
+    F for a deformed state, timed per call."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import sph, synthetic
+    X = synthetic.jittered_lattice((128, 128, 128), 1.0, 0.0, dtype=np.float32, seed=1)
+    n = len(X)
+    h = 1.5
+    pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=np.float32(2 * h), n_points=n)
+    Xd = synthetic.to_device_coords(X, dev)
+    pn.initialize(pre, Xd, Xd)
+    nnz = int(pre.neighbor_lists.col_idx.numel())
+    vol = torch.full((n,), 1.0, dtype=torch.float32, device=dev)
+    k = max(2, min(args.steps, 10))
+    corr_ms = _timed(lambda: sph.tlsph_correction_matrix(pre, Xd, vol, h), k)
+    L = sph.tlsph_correction_matrix(pre, Xd, vol, h)
+    x = Xd * 1.01 + 0.002 * torch.sin(Xd)
+    grad_ms = _timed(lambda: sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L), k)
+    F = sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L)
+    return {
+        "metric": "TLSPH deformation gradient time (128^3 lattice, r = 3)", "value": grad_ms, "unit": "ms",
+        "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": grad_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (lattice, smooth deformation)",
+        "config": {"workload": "tlsph-128cube-3d-f32: 128^3 lattice, spacing 1, r = 3, h = 1.5, Wendland C2",
+                   "n_points": n},
+        "nnz": nnz, "deformation_grad_ms": grad_ms, "pairs_per_s": nnz / (grad_ms * 1e-3),
+        "correction_matrix_ms": corr_ms, "F_finite": bool(torch.isfinite(F).all().item()),
     }
 
 

@@ -271,16 +271,17 @@ int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
  * correction matrix, run by benchmarks/smoothed_particle_hydrodynamics.jl:127-146 through
  * foreach_point_neighbor):
  *     G_a = sum_{b in list(a), |X_a - X_b| >= sqrt(eps)} V_b (x_b - x_a) (grad_a W(X_a - X_b))^T
- *     out_a = G_a L_a  (L_a given), or G_a  (L == NULL)
- * With x = X and L == NULL, out is the correction matrix M (symmetric); with L = M^-1 an affine
- * deformation x = A X + c gives out_a = A exactly.  X, x: D x N; volume (m / rho0) per point;
+ *     out_a = G_a L_a  (L_a given), G_a^-1  (invert != 0, L == NULL), or G_a
+ * With x = X, G is the symmetric correction matrix M, and invert gives L = M^-1 (closed-form
+ * adjugate over determinant); with that L an affine deformation x = A X + c gives out_a = A
+ * exactly.  X, x: D x N; volume (m / rho0) per point;
  * L, out: D x D per point, column-major (Julia's Array{T,3}); one floating type for all
  * arrays (dtype).  Rows selected like pn_grid_update's points.  Enqueue only. */
 int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
                                    int stored_base, int ndims, int dtype, int kernel, double h,
                                    const void* X, const void* x, const void* volume, const void* L,
-                                   int64_t sel_first, int64_t sel_count, const int32_t* sel_list,
-                                   void* out, void* stream);
+                                   int invert, int64_t sel_first, int64_t sel_count,
+                                   const int32_t* sel_list, void* out, void* stream);
 
 /* pn_list_sph_density of a PrecomputedNeighborhoodSearch with a PeriodicBox: a stored pair
  * with d2 > search_radius^2 uses its minimum image (compute_periodic_distance,

@@ -393,14 +393,15 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
 
 int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
                                    int ndims, int dtype, int kernel, double h, const void* X, const void* x,
-                                   const void* volume, const void* L, int64_t sel_first, int64_t sel_count,
-                                   const int32_t* sel_list, void* out, void* stream) {
+                                   const void* volume, const void* L, int invert, int64_t sel_first,
+                                   int64_t sel_count, const int32_t* sel_list, void* out, void* stream) {
     if (!row_ptr || !X || !x || !volume || !out) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the TLSPH kernels are defined for 2D and 3D");
     if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
         return fail(PN_ERR_ARGUMENT, "unknown smoothing kernel");
     if (!(h > 0.0)) return fail(PN_ERR_ARGUMENT, "smoothing length must be positive");
     if (dtype != PN_FLOAT32 && dtype != PN_FLOAT64) return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    if (invert && L) return fail(PN_ERR_ARGUMENT, "invert needs L == NULL");
     if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
         return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
     if (sel_count <= 0) return PN_OK;
@@ -411,7 +412,8 @@ int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_id
         auto kern = pn::SmoothingKernel<T, D>::make(kernel, h);
         hipLaunchKernelGGL((pn::k_list_tlsph_grad<D, T, pn::SmoothingKernel<T, D>>), dim3(nblocks), dim3(256), 0,
                            (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
-                           (const T*)X, (const T*)x, (const T*)volume, kern, (const T*)L, (T*)out, nblocks);
+                           (const T*)X, (const T*)x, (const T*)volume, kern, (const T*)L, invert ? 1 : 0, (T*)out,
+                           nblocks);
         PN_CHECK_LAUNCH();
         return PN_OK;
     };

@@ -300,8 +300,9 @@ __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict_
 // foreach_point_neighbor over the precomputed lists of the initial coordinates):
 //     G_a = sum_{b in list(a), |X_ab| >= sqrt(eps)} V_b (x_b - x_a) (grad_a W(X_ab))^T,
 //     grad_a W(X_ab) = W'(|X_ab|) X_ab / |X_ab|,   X_ab = X_a - X_b,
-// out_a = G_a L_a (L given) or G_a.  With x = X this is the (symmetric) kernel-gradient
-// correction matrix M_a, and L = M^-1 makes G reproduce affine maps exactly.  Sixteen lanes
+// out_a = G_a L_a (L given), G_a^-1 (invert; closed-form adjugate over determinant) or G_a.
+// With x = X this is the (symmetric) kernel-gradient correction matrix M_a, and L = M^-1
+// (invert) makes G reproduce affine maps exactly.  Sixteen lanes
 // per row as in k_list_density; each lane keeps its D x D partial sum, the butterfly over the
 // 16 lanes has a fixed order.  X, x: D x N; out and L: D x D per point, column-major.
 template <int D, class T, class K>
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
                                                          int64_t sel_first, const int32_t* __restrict__ sel_list,
                                                          int64_t n_sel, const T* __restrict__ X, const T* __restrict__ x,
                                                          const T* __restrict__ vol, K kern, const T* __restrict__ L,
-                                                         T* __restrict__ out, int nblocks) {
+                                                         int invert, T* __restrict__ out, int nblocks) {
     const int sub = threadIdx.x & (LIST_LANES - 1);
     const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
     if (a >= n_sel) return;  // whole 16-lane groups leave together
@@ -388,6 +389,36 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
                 for (int k = 1; k < D; ++k) v = fma(g[r][k], l[k + D * c], v);
                 o[r + D * c] = v;
             }
+        }
+    } else if (invert) {
+        T inv[D][D];
+        if constexpr (D == 2) {
+            const T rdet = (T)1 / (g[0][0] * g[1][1] - g[0][1] * g[1][0]);
+            inv[0][0] = g[1][1] * rdet;
+            inv[0][1] = -g[0][1] * rdet;
+            inv[1][0] = -g[1][0] * rdet;
+            inv[1][1] = g[0][0] * rdet;
+        } else {
+            // cofactors: inv[r][c] = C[c][r] / det
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                    inv[r][c] = g[r1][c1] * g[r2][c2] - g[r1][c2] * g[r2][c1];
+                }
+            }
+            const T rdet = (T)1 / (g[0][0] * inv[0][0] + g[0][1] * inv[1][0] + g[0][2] * inv[2][0]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) inv[r][c] *= rdet;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+#pragma unroll
+            for (int r = 0; r < D; ++r) o[r + D * c] = inv[r][c];
         }
     } else {
 #pragma unroll

@@ -107,11 +107,16 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
             cl = FullGridCellList(min_corner=pb.min_corner, max_corner=pb.max_corner, search_radius=r)
             return GridNeighborhoodSearch(self.ndims, search_radius=r, n_points=self.n_points,
                                           periodic_box=pb, cell_list=cl, update_strategy=self.update_strategy)
-        both = torch.cat([x, y], dim=1) if x.shape[1] and y.shape[1] else (x if x.shape[1] else y)
-        if both.shape[1] == 0:
-            both = x
-        lo = both.min(dim=1).values.cpu().numpy()
-        hi = both.max(dim=1).values.cpu().numpy()
+        sets = [t for t in ((x,) if _device.same_storage(x, y) else (x, y)) if t.shape[1]]
+        if sets:
+            # row-major D x N copies reduce along contiguous rows (a reduction over the
+            # strided point dimension runs on a few workgroups only)
+            ext = [torch.aminmax(t.contiguous(), dim=1) for t in sets]
+            lo = torch.stack([e.min for e in ext]).amin(dim=0).cpu().numpy()
+            hi = torch.stack([e.max for e in ext]).amax(dim=0).cpu().numpy()
+        else:
+            import numpy as np
+            lo = hi = np.zeros(self.ndims, dtype=_device.np_dtype(x))
         cl = FullGridCellList(min_corner=lo - r, max_corner=hi + r, search_radius=r)
         return GridNeighborhoodSearch(self.ndims, search_radius=r, n_points=self.n_points,
                                       cell_list=cl, update_strategy=self.update_strategy)

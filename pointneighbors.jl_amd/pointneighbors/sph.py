@@ -155,7 +155,7 @@ def tait_eos(rho, rho0, c, gamma=7.0):
 
 
 def tlsph_deformation_gradient(nhs, X, x, volume, smoothing_length, kernel="wendland_c2", correction=None,
-                               points=None):
+                               points=None, _invert=False):
     """TLSPH deformation gradient over the stored lists of `nhs`, a PrecomputedNeighborhoodSearch
     initialized with the initial coordinates X (TrixiParticles 0.5 calc_deformation_grad!, as run by
     benchmarks/smoothed_particle_hydrodynamics.jl:127-146):
@@ -187,8 +187,8 @@ def tlsph_deformation_gradient(nhs, X, x, volume, smoothing_length, kernel="wend
     first, count, ptr, keep = _device.selection(points, n, X.device)
     check(load().pn_list_tlsph_deformation_grad(
         _vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, D, dtype_code(_device.np_dtype(X)), _kernel_code(kernel),
-        float(smoothing_length), _vp(X), _vp(x), _vp(volume), _vp(L) if L is not None else None, first, count,
-        ctypes.c_void_p(ptr) if ptr else None, _vp(out),
+        float(smoothing_length), _vp(X), _vp(x), _vp(volume), _vp(L) if L is not None else None,
+        1 if _invert else 0, first, count, ctypes.c_void_p(ptr) if ptr else None, _vp(out),
         ctypes.c_void_p(_device.current_stream_handle(X.device))), "tlsph_deformation_gradient")
     del keep
     return out.transpose(1, 2)
@@ -196,11 +196,9 @@ def tlsph_deformation_gradient(nhs, X, x, volume, smoothing_length, kernel="wend
 
 def tlsph_correction_matrix(nhs, X, volume, smoothing_length, kernel="wendland_c2"):
     """Kernel-gradient correction L_a = M_a^-1, M_a = sum_b V_b (X_b - X_a) (grad_a W(X_ab))^T
-    (the deformation gradient of the undeformed body).  M is a HIP list traversal; the D x D
-    inverses are a batched torch call."""
-    import torch
-    M = tlsph_deformation_gradient(nhs, X, X, volume, smoothing_length, kernel)
-    return torch.linalg.inv(M)
+    (the deformation gradient of the undeformed body), one HIP list traversal with a closed-form
+    D x D inverse per point."""
+    return tlsph_deformation_gradient(nhs, X, X, volume, smoothing_length, kernel, _invert=True)
 
 
 def tait_pressure_term(rho, rho0, c, gamma=7.0, out=None):

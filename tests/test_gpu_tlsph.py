@@ -41,6 +41,9 @@ def test_matches_oracle(orc, shape, kernel, dt, tol):
     oM = orc.tlsph_grad(X, X, lists, V, h, kernel)
     assert np.abs(M.cpu().numpy() - oM).max() <= tol * np.abs(oM).max()
     L = torch.linalg.inv(M)
+    # the kernel's closed-form inverse equals the batched LU inverse
+    Lk = sph.tlsph_correction_matrix(p, Xd, Vd, h, kernel)
+    assert np.abs(Lk.cpu().numpy() - L.cpu().numpy()).max() <= 100 * tol * np.abs(L.cpu().numpy()).max()
     F = sph.tlsph_deformation_gradient(p, Xd, dev(x), Vd, h, kernel, correction=L)
     oF = orc.tlsph_grad(X, x, lists, V, h, kernel, L=L.cpu().numpy().astype(np.float64))
     assert np.abs(F.cpu().numpy() - oF).max() <= 10 * tol * np.abs(oF).max()

@@ -266,6 +266,17 @@ int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
                         int64_t sel_count, const int32_t* sel_list, void* rho, int accumulate,
                         void* stream);
 
+/* CSR lists (pn_csr_fill) -> the reference's DynamicVectorOfVectors{Int32} backend
+ * (src/vector_of_vectors.jl:3-31; PrecomputedNeighborhoodSearch's `backend` and
+ * `transpose_backend`, src/nhs_precomputed.jl:40-66).  lengths[i] = length of row i.  Entry k of
+ * row i (k < max_inner) goes to backend[k + max_inner * i] (Array{Int32,2}(max_inner, N)), or
+ * with transpose != 0 to backend[i + ld * k] (the row-major array behind the reference's
+ * PermutedDimsArray; ld = max_outer_length >= n_rows).  Slots past a row's length are not
+ * written.  The caller checks max_row <= max_inner (pn_csr_count reports max_row): longer rows
+ * are the reference's list overflow.  Enqueue only. */
+int pn_csr_to_dvov(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int max_inner,
+                   int transpose, int64_t ld, int32_t* backend, int32_t* lengths, void* stream);
+
 /* TLSPH deformation gradient over the stored lists of a PrecomputedNeighborhoodSearch built on
  * the initial coordinates X (TrixiParticles 0.5 calc_deformation_grad! and the kernel-gradient
  * correction matrix, run by benchmarks/smoothed_particle_hydrodynamics.jl:127-146 through

@@ -391,6 +391,24 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
     });
 }
 
+int pn_csr_to_dvov(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int max_inner, int transpose,
+                   int64_t ld, int32_t* backend, int32_t* lengths, void* stream) {
+    if (!row_ptr || !backend || !lengths) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (max_inner <= 0) return fail(PN_ERR_ARGUMENT, "max_inner must be positive");
+    if (transpose && ld < n_rows) return fail(PN_ERR_ARGUMENT, "ld must be at least n_rows");
+    if (n_rows <= 0) return PN_OK;
+    if (transpose) {
+        hipLaunchKernelGGL(pn::k_csr_to_dvov_rows, dim3(blocks_for(n_rows, 256)), dim3(256), 0, (hipStream_t)stream,
+                           row_ptr, col_idx, n_rows, max_inner, ld, backend, lengths);
+    } else {
+        const int nblocks = (int)std::min<int64_t>(n_rows, 65536);
+        hipLaunchKernelGGL(pn::k_csr_to_dvov_cols, dim3(nblocks), dim3(pn::WAVE), 0, (hipStream_t)stream, row_ptr,
+                           col_idx, n_rows, max_inner, backend, lengths, nblocks);
+    }
+    PN_CHECK_LAUNCH();
+    return PN_OK;
+}
+
 int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
                                    int ndims, int dtype, int kernel, double h, const void* X, const void* x,
                                    const void* volume, const void* L, int invert, int64_t sel_first,

@@ -208,6 +208,37 @@ static __global__ __launch_bounds__(WAVE) void k_list_index_sum(const int32_t* _
     if (threadIdx.x == 0) out[i] = accumulate ? out[i] + s : s;
 }
 
+// CSR lists -> the reference's DynamicVectorOfVectors backend (src/vector_of_vectors.jl:3-31,
+// src/nhs_precomputed.jl:40-66): lengths[i] = row length; entry k of row i (k < max_inner) at
+// out[k + max_inner * i] (columns per point), or with transpose at out[i + ld * k] (the k-th
+// neighbours of all points contiguous, transpose_backend = true; ld = max_outer_length).
+// Slots past a row's length are not written (the reference leaves them undefined).
+// Non-transposed: one wavefront per row, coalesced over the row; transposed: one thread per
+// row, coalesced over rows for every k.
+static __global__ __launch_bounds__(WAVE) void k_csr_to_dvov_cols(const int32_t* __restrict__ row_ptr,
+                                                           const int32_t* __restrict__ col, int64_t n_rows,
+                                                           int max_inner, int32_t* __restrict__ out,
+                                                           int32_t* __restrict__ lengths, int nblocks) {
+    for (int64_t i = blockIdx.x; i < n_rows; i += nblocks) {
+        const int b = row_ptr[i], n = row_ptr[i + 1] - b;
+        if (threadIdx.x == 0) lengths[i] = n;
+        const int m = min(n, max_inner);
+        for (int k = threadIdx.x; k < m; k += WAVE) out[(int64_t)max_inner * i + k] = col[b + k];
+    }
+}
+
+static __global__ __launch_bounds__(256) void k_csr_to_dvov_rows(const int32_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col, int64_t n_rows,
+                                                          int max_inner, int64_t ld, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ lengths) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rows) return;
+    const int b = row_ptr[i], n = row_ptr[i + 1] - b;
+    lengths[i] = n;
+    const int m = min(n, max_inner);
+    for (int k = 0; k < m; ++k) out[i + ld * k] = col[b + k];
+}
+
 // Periodic box of a list traversal: compute_periodic_distance
 // (src/neighborhood_search.jl:428-436) applies when on != 0.
 template <class TR>

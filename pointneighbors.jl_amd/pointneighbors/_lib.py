@@ -89,6 +89,7 @@ SIGNATURES = {
     "pn_displace_uniform": (_c_int, [_vp, _c_i64, _c_int, _c_dbl, ctypes.c_uint64, _vp, _vp]),
     "pn_list_sph_density": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp,
                                      _vp, _c_i64, _c_i64, _vp, _vp, _c_int, _vp]),
+    "pn_csr_to_dvov": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_i64, _vp, _vp, _vp]),
     "pn_list_tlsph_deformation_grad": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp,
                                                 _vp, _vp, _c_int, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pn_list_sph_density_periodic": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl,

@@ -52,6 +52,35 @@ class NeighborLists:
     def to_host(self):
         return self.row_ptr.cpu().numpy(), self.col_idx.cpu().numpy()
 
+    def to_dvov(self, max_neighbors=None, transpose_backend=False, max_outer_length=None):
+        """The reference's DynamicVectorOfVectors{Int32} layout (src/vector_of_vectors.jl:3-31) of
+        these lists, on the device: (backend, lengths).  backend is an (N, max_neighbors) int32
+        tensor whose row i holds point i's neighbours (Julia's Array{Int32,2}(max_neighbors, N));
+        with transpose_backend it is a (max_neighbors, max_outer_length) tensor whose row k holds
+        the k-th neighbours of all points (the row-major array behind the reference's
+        PermutedDimsArray).  Slots past a list's length are zero here."""
+        import torch
+        max_neighbors = int(self.max_neighbors if max_neighbors is None else max_neighbors)
+        n = self.n
+        if max_neighbors <= 0:
+            raise ArgumentError("max_neighbors must be positive")
+        longest = int(self.lengths().max().item()) if n else 0
+        if longest > max_neighbors:
+            raise PointNeighborsError(f"a neighbor list has {longest} entries, more than max_neighbors = "
+                                      f"{max_neighbors}")
+        dev = self.row_ptr.device
+        ld = n if max_outer_length is None else int(max_outer_length)
+        if ld < n:
+            raise ArgumentError("max_outer_length must be at least the number of points")
+        shape = (max_neighbors, ld) if transpose_backend else (n, max_neighbors)
+        backend = torch.zeros(shape, dtype=torch.int32, device=dev)
+        lengths = torch.zeros(n, dtype=torch.int32, device=dev)
+        check(load().pn_csr_to_dvov(ctypes.c_void_p(self.row_ptr.data_ptr()), ctypes.c_void_p(self.col_idx.data_ptr()),
+                                    n, max_neighbors, 1 if transpose_backend else 0, ld,
+                                    ctypes.c_void_p(backend.data_ptr()), ctypes.c_void_p(lengths.data_ptr()),
+                                    ctypes.c_void_p(_device.current_stream_handle(dev))), "to_dvov")
+        return backend, lengths
+
 
 class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
     """PrecomputedNeighborhoodSearch{NDIMS}(; search_radius = 0.0, n_points = 0,

@@ -420,3 +420,29 @@ def test_precomputed_long_rows(orc, n_pts, extent, longest):
     assert lens.max() > longest and (lens <= 64).any() and ((lens > 128) & (lens <= 256)).any()
     rp, cols = p.neighbor_lists.to_host()
     assert np.array_equal(rp.astype(np.int64), orp) and np.array_equal(cols, ocols)
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_dvov_layouts(orc, transpose):
+    """CSR lists exported to the reference's DynamicVectorOfVectors backend (plain and
+    transpose_backend): every list and length equals the CSR row, in both layouts."""
+    coords = cloud((23, 21, 19), 5, dtype=np.float32)
+    r = np.float32(2.2)
+    nhs = make_nhs(coords, r, coords.min(axis=0) - r, coords.max(axis=0) + r)
+    x = dev(coords)
+    p = pn.PrecomputedNeighborhoodSearch(3, search_radius=r, n_points=len(coords), update_neighborhood_search=nhs,
+                                         max_neighbors=128)
+    pn.initialize(p, x, x)
+    rp, cols = p.neighbor_lists.to_host()
+    longest = int(np.diff(rp).max())
+    backend, lengths = p.neighbor_lists.to_dvov(max_neighbors=longest + 3, transpose_backend=transpose,
+                                                max_outer_length=len(coords) + 7 if transpose else None)
+    b = backend.cpu().numpy()
+    if transpose:
+        assert b.shape == (longest + 3, len(coords) + 7)
+        b = b.T
+    assert np.array_equal(lengths.cpu().numpy(), np.diff(rp))
+    for i in range(len(coords)):
+        assert np.array_equal(b[i, :rp[i + 1] - rp[i]], cols[rp[i]:rp[i + 1]])
+    with pytest.raises(pn.PointNeighborsError):
+        p.neighbor_lists.to_dvov(max_neighbors=longest - 1)

@@ -102,9 +102,9 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     constexpr int FOFF = kPackF ? 1 : 0;
     constexpr int NFL = NF - FOFF;  // fields kept in their own LDS planes
     __shared__ TR s_f[NFL > 0 ? NFL : 1][(!kLight && NFL > 0) ? CH : 1];
-    // pair list; a replay copies a cell's records (up to PC_CELL_MAX entries) here
-    // Entries are candidate slots of the chunk: one byte when a chunk holds at most 256 and no
-    // pair cache is involved (its records are 16-bit).
+    // Pair list: candidate slots of the chunk, one byte each when a chunk holds at most 256 and
+    // no pair cache is involved; a replay copies a cell's 16-bit records (up to PC_CELL_MAX
+    // entries) here.
     using PairT = typename std::conditional<(PCM == 0 && CH <= 256), uint8_t, uint16_t>::type;
     constexpr int PAIR_WORDS = (PCM == 0 ? (PAIR_CAP + WAVE) * (int)sizeof(PairT) : 2 * PC_CELL_MAX) / 8;
     __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];

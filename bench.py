@@ -469,35 +469,61 @@ def bench_csr(args, dev):
 
 
 def bench_tlsph(args, dev):
-    """TLSPH deformation gradient (TrixiParticles calc_deformation_grad!, the reference's
-    benchmark_tlsph_deformation_grad) on a 128^3 Float32 lattice, spacing 1, search radius 3
-    (h = 1.5): precomputed lists of the initial coordinates, the correction matrix once, then
-    F for a deformed state, timed per call."""
+    """TLSPH on the cloud of the reference's published TLSPH runs: 101^3 = 1,030,301 points, a
+    unit lattice jittered twice by N(0, 0.05^2) and stably sorted by the cells (size 3) of the
+    first jitter, first dimension slowest (test/point_cloud.jl:4-57 as driven by
+    benchmarks/run_benchmarks.jl:60-113; lattice units, not rescaled to a unit box), search radius 3, h = r/2, Wendland C2, the material of
+    setup_tlsph (benchmarks/smoothed_particle_hydrodynamics.jl:146-187: rho 1000, m 0.1,
+    E 1.4e6, nu 0.4, Ganzenmueller penalty alpha 0.1), Float32, precomputed lists of the initial
+    coordinates.  Times the two loops the reference times: interact_structure_structure!
+    (benchmark_tlsph, accumulating into dv) and calc_deformation_grad!
+    (benchmark_tlsph_deformation_grad), both per call with the lists resident."""
     import torch
     import pointneighbors as pn
     from pointneighbors import sph, synthetic
-    X = synthetic.jittered_lattice((128, 128, 128), 1.0, 0.0, dtype=np.float32, seed=1)
+    side = 101
+    r = np.float32(3.0)
+    rng = np.random.default_rng(1)
+    X = synthetic.jittered_lattice((side,) * 3, 1.0, 0.0, dtype=np.float64)
+    X = X + 0.05 * rng.standard_normal(X.shape)
+    cells = np.floor(X / float(r)).astype(np.int64)  # sorted by these cells, first dimension slowest
+    X = X + 0.05 * rng.standard_normal(X.shape)
+    X = X[np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))].astype(np.float32)
     n = len(X)
-    h = 1.5
-    pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=np.float32(2 * h), n_points=n)
+    h = float(r) / 2
+    pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=r, n_points=n)
     Xd = synthetic.to_device_coords(X, dev)
     pn.initialize(pre, Xd, Xd)
     nnz = int(pre.neighbor_lists.col_idx.numel())
-    vol = torch.full((n,), 1.0, dtype=torch.float32, device=dev)
+    rho0, m0, E, nu, alpha = 1000.0, 0.1, 1.4e6, 0.4, 0.1
+    mass = torch.full((n,), m0, dtype=torch.float32, device=dev)
+    rho = torch.full((n,), rho0, dtype=torch.float32, device=dev)
+    vol = mass / rho
     k = max(2, min(args.steps, 10))
     corr_ms = _timed(lambda: sph.tlsph_correction_matrix(pre, Xd, vol, h), k)
     L = sph.tlsph_correction_matrix(pre, Xd, vol, h)
     x = Xd * 1.01 + 0.002 * torch.sin(Xd)
     grad_ms = _timed(lambda: sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L), k)
     F = sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L)
+    pk1_ms = _timed(lambda: sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L), k)
+    Pr = sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L)
+    dv = sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E, penalty_alpha=alpha)
+    inter_ms = _timed(lambda: sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E,
+                                                    penalty_alpha=alpha, dv=dv), k)
+    h100_interact_ms, h100_grad_ms = 3.801, 0.825  # benchmarks/plot_benchmarks.jl:76,84
     return {
-        "metric": "TLSPH deformation gradient time (128^3 lattice, r = 3)", "value": grad_ms, "unit": "ms",
-        "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": grad_ms, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (lattice, smooth deformation)",
-        "config": {"workload": "tlsph-128cube-3d-f32: 128^3 lattice, spacing 1, r = 3, h = 1.5, Wendland C2",
+        "metric": "TLSPH interaction time (benchmark_tlsph, 1,030,301 points)", "value": inter_ms, "unit": "ms",
+        "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": inter_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": inter_ms / h100_interact_ms, "dtype": "f32",
+        "data": "synthetic (jittered lattice, smooth deformation)",
+        "config": {"workload": "tlsph-101cube-3d-f32: 101^3 lattice, 2 x N(0, 0.05^2) jitter, cell-sorted, r = 3, h = 1.5, "
+                               "Wendland C2, rho 1000, m 0.1, E 1.4e6, nu 0.4, penalty alpha 0.1",
                    "n_points": n},
-        "nnz": nnz, "deformation_grad_ms": grad_ms, "pairs_per_s": nnz / (grad_ms * 1e-3),
-        "correction_matrix_ms": corr_ms, "F_finite": bool(torch.isfinite(F).all().item()),
+        "nnz": nnz, "interaction_ms": inter_ms, "interaction_pairs_per_s": nnz / (inter_ms * 1e-3),
+        "deformation_grad_ms": grad_ms, "deformation_grad_pairs_per_s": nnz / (grad_ms * 1e-3),
+        "speedup_vs_h100": {"interaction": h100_interact_ms / inter_ms, "deformation_grad": h100_grad_ms / grad_ms},
+        "correction_matrix_ms": corr_ms, "pk1_corrected_ms": pk1_ms,
+        "finite": bool(torch.isfinite(F).all().item() and torch.isfinite(dv).all().item()),
     }
 
 

@@ -294,6 +294,31 @@ int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_id
                                    int invert, int64_t sel_first, int64_t sel_count,
                                    const int32_t* sel_list, void* out, void* stream);
 
+/* TLSPH interaction forces over the same stored lists (TrixiParticles 0.5
+ * interact_structure_structure!, the loop the reference times in benchmark_tlsph,
+ * benchmarks/smoothed_particle_hydrodynamics.jl:115-121 with the system of :146-187):
+ *     dv_a (+)= sum_{b in list(a), |X_ab| >= sqrt(eps)} m_b (Pr_a + Pr_b) grad_a W(X_ab) + f_ab / m_a
+ *     f_ab = (alpha / 2) V_a V_b W(|X_ab|) / |X_ab|^2 E delta_ab x_ab / |x_ab|,
+ *     delta_ab = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab|,  V = m / rho
+ * (the penalty force of Ganzenmueller 2015, PenaltyForceGanzenmueller; skipped when
+ * penalty_alpha == 0, then x, rho and F may be NULL).  X (initial), x (current): D x N;
+ * mass and rho (material density) per point; pk1_rho2 = pk1_corrected / rho^2 and F: D x D per
+ * point, column-major.  dv: D x N.  One floating type for all arrays.  Enqueue only. */
+int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
+                              int stored_base, int ndims, int dtype, int kernel, double h,
+                              const void* X, const void* x, const void* mass, const void* rho,
+                              const void* pk1_rho2, const void* F, double young_modulus,
+                              double penalty_alpha, int64_t sel_first, int64_t sel_count,
+                              const int32_t* sel_list, void* dv, int accumulate, void* stream);
+
+/* Corrected first Piola-Kirchhoff stress over rho^2 of a Saint Venant-Kirchhoff material
+ * (TrixiParticles compute_pk1_corrected!, run once by the set-up of benchmark_tlsph,
+ * benchmarks/smoothed_particle_hydrodynamics.jl:186): E = (F^T F - I) / 2,
+ * S = lame_lambda tr(E) I + 2 lame_mu E, out = F S L / rho^2 (L == NULL: identity).  F, L, out:
+ * D x D per point, column-major; rho per point.  Enqueue only. */
+int pn_tlsph_pk1_rho2(const void* F, const void* L, const void* rho, int64_t n, int ndims, int dtype,
+                      double lame_lambda, double lame_mu, void* out, void* stream);
+
 /* pn_list_sph_density of a PrecomputedNeighborhoodSearch with a PeriodicBox: a stored pair
  * with d2 > search_radius^2 uses its minimum image (compute_periodic_distance,
  * src/neighborhood_search.jl:428-436, called at src/nhs_precomputed.jl:236-238).

@@ -74,6 +74,10 @@ def lib():
         L.orc_sph_density.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp]
         L.orc_tlsph_grad.restype = None
         L.orc_tlsph_grad.argtypes = [ci, ci, dbl, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_tlsph_force.restype = None
+        L.orc_tlsph_force.argtypes = [ci, ci, dbl, vp, vp, i64, vp, vp, vp, vp, vp, vp, dbl, vp]
+        L.orc_tlsph_pk1_rho2.restype = None
+        L.orc_tlsph_pk1_rho2.argtypes = [ci, vp, vp, vp, i64, dbl, dbl, vp]
         L.orc_sph_pressure_accel.restype = None
         L.orc_sph_pressure_accel.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp, vp]
         L.orc_pair_energy.restype = None
@@ -332,6 +336,35 @@ def tlsph_grad(X_nd, x_nd, lists, volume, h, kernel="wendland_c2", L=None):
     Lp = None if L is None else np.ascontiguousarray(L, dtype=np.float64)
     lib().orc_tlsph_grad(nd, KERNELS[kernel], float(h), _ptr(X), _ptr(x), n, _ptr(rp), _ptr(cols), _ptr(v),
                          None if Lp is None else _ptr(Lp), _ptr(out))
+    return out
+
+
+def tlsph_force(X_nd, x_nd, lists, mass, rho, pk1_rho2, h, kernel="wendland_c2", F=None, young_modulus=0.0,
+                penalty_alpha=0.0):
+    """TLSPH interaction dv (N, D) (oracle restatement); pk1_rho2 and F are (N, D, D) with
+    [a] the matrix of point a; the penalty term needs F and penalty_alpha > 0."""
+    X = np.ascontiguousarray(X_nd, dtype=np.float64)
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    P = np.ascontiguousarray(pk1_rho2, dtype=np.float64)
+    Fp = None if (F is None or penalty_alpha == 0.0) else np.ascontiguousarray(F, dtype=np.float64)
+    n, nd = X.shape
+    dv = np.zeros((n, nd))
+    lib().orc_tlsph_force(nd, KERNELS[kernel], float(h), _ptr(X), _ptr(x), n, _ptr(rp), _ptr(cols), _ptr(m), _ptr(r),
+                          _ptr(P), _ptr(Fp), 0.5 * penalty_alpha * young_modulus, _ptr(dv))
+    return dv
+
+
+def tlsph_pk1_rho2(F, rho, lame_lambda, lame_mu, L=None):
+    """Corrected SVK PK1 / rho^2, (N, D, D) (oracle restatement)."""
+    Fa = np.ascontiguousarray(F, dtype=np.float64)
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    Lp = None if L is None else np.ascontiguousarray(L, dtype=np.float64)
+    out = np.zeros_like(Fa)
+    lib().orc_tlsph_pk1_rho2(Fa.shape[1], _ptr(Fa), _ptr(Lp), _ptr(r), Fa.shape[0], float(lame_lambda),
+                             float(lame_mu), _ptr(out))
     return out
 
 

@@ -24,6 +24,13 @@
 //   the d^2 evaluation order parity unpinned against the reference (StaticArrays is
 //                            not vendored); pinned to the stated reading by a numpy
 //                            step-rounded evaluation on adversarial pairs
+//   orc_tlsph_force (stress)  parity unpinned against TrixiParticles (absent); pinned by the
+//                            exact discrete energy identity m_a dv_a = -dE/dx_a with
+//                            E = sum_c V_c Psi_SVK(F_c), F from orc_tlsph_grad (finite
+//                            differences), two-point tension/compression signs, momentum
+//   orc_tlsph_force (penalty) parity unpinned; pinned by momentum conservation, zero force
+//                            under affine maps with F = A, a compressed pair repelling
+//   orc_tlsph_pk1_rho2       pinned by the same energy identity (P = dPsi/dF)
 //   orc_sph_* / orc_energy_* parity unpinned against the reference (the physics lives in
 //                            TrixiParticles, absent); pinned by closed-form single-pair
 //                            values, symmetry, and forward-mode duals vs finite differences
@@ -857,6 +864,92 @@ void orc_tlsph_grad(int nd, int kernel, double h, const double* X, const double*
                     v = g[i][j];
                 }
                 o[i * nd + j] = v;
+            }
+    }
+}
+
+// TLSPH interaction (TrixiParticles interact_structure_structure!, the loop of the reference's
+// benchmark_tlsph, benchmarks/smoothed_particle_hydrodynamics.jl:115-121): stress term
+// m_b (Pr_a + Pr_b) grad_a W(X_ab) plus, when F != nullptr, the Ganzenmueller penalty
+// (alpha/2) V_a V_b W / |X_ab|^2 E delta x_ab / |x_ab| / m_a with
+// delta = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab|.  hae = alpha E / 2.  Pr, F row-major
+// D x D per point; X, x, dv: N x D.
+void orc_tlsph_force(int nd, int kernel, double h, const double* X, const double* x, int64_t n, const int64_t* rp,
+                     const int32_t* cols, const double* mass, const double* rho, const double* Pr, const double* F,
+                     double hae, double* dv) {
+    const Kern<double> k{kernel, nd, h};
+    const double tiny = 1.4901161193847656e-8;
+    for (int64_t a = 0; a < n; ++a) {
+        double acc[3] = {0, 0, 0};
+        for (int64_t q = rp[a]; q < rp[a + 1]; ++q) {
+            const int64_t b = cols[q];
+            double Xab[3] = {0, 0, 0}, d2 = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                Xab[d] = X[a * nd + d] - X[b * nd + d];
+                d2 += Xab[d] * Xab[d];
+            }
+            const double r = std::sqrt(d2);
+            if (r < tiny) continue;
+            double gw[3];
+            for (int d = 0; d < nd; ++d) gw[d] = k.dw(r) * Xab[d] / r;
+            for (int i = 0; i < nd; ++i) {
+                double v = 0.0;
+                for (int j = 0; j < nd; ++j) v += (Pr[a * nd * nd + i * nd + j] + Pr[b * nd * nd + i * nd + j]) * gw[j];
+                acc[i] += mass[b] * v;
+            }
+            if (!F) continue;
+            double xab[3] = {0, 0, 0}, c2 = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                xab[d] = x[a * nd + d] - x[b * nd + d];
+                c2 += xab[d] * xab[d];
+            }
+            const double cd = std::sqrt(c2);
+            double delta = 0.0;
+            for (int i = 0; i < nd; ++i) {
+                double e = -2.0 * xab[i];
+                for (int j = 0; j < nd; ++j) e += (F[a * nd * nd + i * nd + j] + F[b * nd * nd + i * nd + j]) * Xab[j];
+                delta += e * xab[i];
+            }
+            delta /= cd;
+            const double Va = mass[a] / rho[a], Vb = mass[b] / rho[b];
+            const double f = hae * Va * Vb * k.w(r) / (r * r) * delta / cd;
+            for (int d = 0; d < nd; ++d) acc[d] += f * xab[d] / mass[a];
+        }
+        for (int d = 0; d < nd; ++d) dv[a * nd + d] = acc[d];
+    }
+}
+
+// Saint Venant-Kirchhoff corrected PK1 over rho^2 (TrixiParticles compute_pk1_corrected!):
+// E = (F^T F - I)/2, S = lambda tr(E) I + 2 mu E, out = F S L / rho^2 (L == nullptr: identity).
+// F, L, out row-major D x D per point.
+void orc_tlsph_pk1_rho2(int nd, const double* F, const double* L, const double* rho, int64_t n, double lambda,
+                        double mu, double* out) {
+    for (int64_t a = 0; a < n; ++a) {
+        const double* f = F + a * nd * nd;
+        double E[3][3], S[3][3], P[3][3], tr = 0.0;
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                double v = 0.0;
+                for (int m = 0; m < nd; ++m) v += f[m * nd + i] * f[m * nd + j];
+                E[i][j] = 0.5 * (v - (i == j ? 1.0 : 0.0));
+            }
+        for (int i = 0; i < nd; ++i) tr += E[i][i];
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) S[i][j] = 2.0 * mu * E[i][j] + (i == j ? lambda * tr : 0.0);
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                double v = 0.0;
+                for (int m = 0; m < nd; ++m) v += f[i * nd + m] * S[m][j];
+                P[i][j] = v;
+            }
+        for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) {
+                double v = P[i][j];
+                if (L) {
+                    v = 0.0;
+                    for (int m = 0; m < nd; ++m) v += P[i][m] * L[a * nd * nd + m * nd + j];
+                }
+                out[a * nd * nd + i * nd + j] = v / (rho[a] * rho[a]);
             }
     }
 }

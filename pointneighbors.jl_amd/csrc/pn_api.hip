@@ -439,6 +439,66 @@ int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_id
     return dtype == PN_FLOAT32 ? run(pn::IntC<3>{}, 0.0f) : run(pn::IntC<3>{}, 0.0);
 }
 
+int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
+                              int ndims, int dtype, int kernel, double h, const void* X, const void* x,
+                              const void* mass, const void* rho, const void* pk1_rho2, const void* F,
+                              double young_modulus, double penalty_alpha, int64_t sel_first, int64_t sel_count,
+                              const int32_t* sel_list, void* dv, int accumulate, void* stream) {
+    if (!row_ptr || !X || !mass || !pk1_rho2 || !dv) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the TLSPH kernels are defined for 2D and 3D");
+    if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
+        return fail(PN_ERR_ARGUMENT, "unknown smoothing kernel");
+    if (!(h > 0.0)) return fail(PN_ERR_ARGUMENT, "smoothing length must be positive");
+    if (dtype != PN_FLOAT32 && dtype != PN_FLOAT64) return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    if (!(penalty_alpha >= 0.0)) return fail(PN_ERR_ARGUMENT, "penalty_alpha must be non-negative");
+    const bool pen = penalty_alpha > 0.0;
+    if (pen && (!x || !rho || !F)) return fail(PN_ERR_ARGUMENT, "the penalty force needs x, rho and F");
+    if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
+        return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
+    if (sel_count <= 0) return PN_OK;
+    const int nblocks = (int)round_up8((sel_count + pn::LIST_ROWS_PER_BLOCK - 1) / pn::LIST_ROWS_PER_BLOCK);
+    auto run = [&](auto dc, auto tv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using T = decltype(tv);
+        using K = pn::SmoothingKernel<T, D>;
+        auto kern = K::make(kernel, h);
+        const T hae = (T)(0.5 * penalty_alpha * young_modulus);
+        if (pen)
+            hipLaunchKernelGGL((pn::k_list_tlsph_force<D, T, K, true>), dim3(nblocks), dim3(256), 0,
+                               (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
+                               (const T*)X, (const T*)x, (const T*)mass, (const T*)rho, (const T*)pk1_rho2,
+                               (const T*)F, hae, kern, (T*)dv, accumulate ? 1 : 0, nblocks);
+        else
+            hipLaunchKernelGGL((pn::k_list_tlsph_force<D, T, K, false>), dim3(nblocks), dim3(256), 0,
+                               (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
+                               (const T*)X, (const T*)x, (const T*)mass, (const T*)rho, (const T*)pk1_rho2,
+                               (const T*)F, hae, kern, (T*)dv, accumulate ? 1 : 0, nblocks);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    };
+    if (ndims == 2) return dtype == PN_FLOAT32 ? run(pn::IntC<2>{}, 0.0f) : run(pn::IntC<2>{}, 0.0);
+    return dtype == PN_FLOAT32 ? run(pn::IntC<3>{}, 0.0f) : run(pn::IntC<3>{}, 0.0);
+}
+
+int pn_tlsph_pk1_rho2(const void* F, const void* L, const void* rho, int64_t n, int ndims, int dtype,
+                      double lame_lambda, double lame_mu, void* out, void* stream) {
+    if (!F || !rho || !out) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the TLSPH kernels are defined for 2D and 3D");
+    if (dtype != PN_FLOAT32 && dtype != PN_FLOAT64) return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    if (n < 0) return fail(PN_ERR_ARGUMENT, "n must be non-negative");
+    if (n == 0) return PN_OK;
+    auto run = [&](auto dc, auto tv) -> int {
+        constexpr int D = decltype(dc)::value;
+        using T = decltype(tv);
+        hipLaunchKernelGGL((pn::k_tlsph_pk1_rho2<D, T>), dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const T*)F, (const T*)L, (const T*)rho, n, (T)lame_lambda, (T)lame_mu, (T*)out);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    };
+    if (ndims == 2) return dtype == PN_FLOAT32 ? run(pn::IntC<2>{}, 0.0f) : run(pn::IntC<2>{}, 0.0);
+    return dtype == PN_FLOAT32 ? run(pn::IntC<3>{}, 0.0f) : run(pn::IntC<3>{}, 0.0);
+}
+
 int pn_list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base, int ndims,
                         int coord_dtype, int radius_dtype, int kernel, double h, const void* x, const void* y,
                         const void* mass, int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* rho,

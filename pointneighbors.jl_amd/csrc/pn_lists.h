@@ -460,4 +460,184 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
     }
 }
 
+// TLSPH interaction over stored lists of the initial coordinates (TrixiParticles'
+// interact_structure_structure!, timed by the reference's benchmark_tlsph at
+// benchmarks/smoothed_particle_hydrodynamics.jl:115-121, set up at :146-187):
+//     dv_a (+)= sum_{b in list(a), |X_ab| >= sqrt(eps)} m_b (Pr_a + Pr_b) grad_a W(X_ab)   [stress]
+//              + f_ab / m_a                                                             [penalty]
+// Pr = corrected first Piola-Kirchhoff stress / rho^2 (D x D per point, column-major),
+// grad_a W = W'(|X_ab|) X_ab / |X_ab|.  The penalty (Ganzenmueller 2015 hourglass control,
+// enabled when PEN) with x_ab = x_a - x_b the current difference, F the deformation gradient:
+//     delta = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab|
+//     f_ab  = (alpha / 2) V_a V_b W(|X_ab|) / |X_ab|^2 * E * delta * x_ab / |x_ab|,  V = m / rho.
+// Both terms are antisymmetric in (a, b): sum_a m_a dv_a = 0 over symmetric lists.  Sixteen
+// lanes per row as in k_list_density; dv: D x N.
+template <int D, class T, class K, bool PEN>
+__global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col, int stored_base,
+                                                          int64_t sel_first, const int32_t* __restrict__ sel_list,
+                                                          int64_t n_sel, const T* __restrict__ X,
+                                                          const T* __restrict__ x, const T* __restrict__ mass,
+                                                          const T* __restrict__ rho, const T* __restrict__ Pr,
+                                                          const T* __restrict__ F, T half_alpha_E, K kern,
+                                                          T* __restrict__ dv, int accumulate, int nblocks) {
+    const int sub = threadIdx.x & (LIST_LANES - 1);
+    const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
+    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    const int64_t i = sel_point(a, sel_first, sel_list);
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    T Xi[D], xi[D], Pi[D * D], Fi[PEN ? D * D : 1];
+#pragma unroll
+    for (int d = 0; d < D; ++d) Xi[d] = X[(int64_t)D * i + d];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) Pi[k] = Pr[(int64_t)D * D * i + k];
+    T pen_a = (T)0;  // (alpha / 2) E V_a / m_a = (alpha / 2) E / rho_a
+    if constexpr (PEN) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) xi[d] = x[(int64_t)D * i + d];
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) Fi[k] = F[(int64_t)D * D * i + k];
+        pen_a = half_alpha_E / rho[i];
+    }
+    const T tiny = (T)1.4901161193847656e-8;  // sqrt(eps(Float64)): coincident points add nothing
+    T acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = (T)0;
+    constexpr int U = PEN ? 2 : 4;  // a neighbour brings D*D + D + 1 values, 2 D*D + 2 D + 2 with PEN
+    for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * U) {
+        int64_t j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + LIST_LANES * u;
+            j[u] = k < e ? (int64_t)col[k] - stored_base : -1;
+        }
+        T Xj[U][D], mj[U], Pj[U][D * D], xj[U][D], rj[U], Fj[U][PEN ? D * D : 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t jj = j[u] >= 0 ? j[u] : 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) Xj[u][d] = X[(int64_t)D * jj + d];
+            mj[u] = mass[jj];
+#pragma unroll
+            for (int k = 0; k < D * D; ++k) Pj[u][k] = Pr[(int64_t)D * D * jj + k];
+            if constexpr (PEN) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) xj[u][d] = x[(int64_t)D * jj + d];
+                rj[u] = rho[jj];
+#pragma unroll
+                for (int k = 0; k < D * D; ++k) Fj[u][k] = F[(int64_t)D * D * jj + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            T Xab[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) Xab[d] = Xi[d] - Xj[u][d];
+            T d2 = Xab[0] * Xab[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
+            const T dist = sqrt(d2);
+            const bool use = j[u] >= 0 && dist >= tiny;
+            const T safe = use ? dist : (T)1;
+            // m_b W'(r) / r, so that m_b grad_a W = s X_ab
+            const T s = use ? mj[u] * kern.dw(safe) / safe : (T)0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                T v = (T)0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) v = fma(Pi[r + D * c] + Pj[u][r + D * c], Xab[c], v);
+                acc[r] = fma(s, v, acc[r]);
+            }
+            if constexpr (PEN) {
+                T xab[D], eps[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) xab[d] = xi[d] - xj[u][d];
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    T v = (T)-2 * xab[r];
+#pragma unroll
+                    for (int c = 0; c < D; ++c) v = fma(Fi[r + D * c] + Fj[u][r + D * c], Xab[c], v);
+                    eps[r] = v;
+                }
+                T c2 = xab[0] * xab[0], dl = eps[0] * xab[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) {
+                    c2 = c2 + xab[d] * xab[d];
+                    dl = fma(eps[d], xab[d], dl);
+                }
+                // V_b W(r) / r^2 * delta / |x_ab| * (alpha / 2) E / rho_a, times x_ab / |x_ab|
+                const T g = use ? pen_a * (mj[u] / rj[u]) * kern.w(safe) / (safe * safe) * dl / c2 : (T)0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) acc[d] = fma(g, xab[d], acc[d]);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = LIST_LANES / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], o, LIST_LANES);
+    }
+    if (sub != 0) return;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        T* o = dv + (int64_t)D * i + d;
+        *o = accumulate ? *o + acc[d] : acc[d];
+    }
+}
+
+// Corrected first Piola-Kirchhoff stress over rho^2 of a Saint Venant-Kirchhoff material
+// (TrixiParticles compute_pk1_corrected!): E = (F^T F - I) / 2, S = lambda tr(E) I + 2 mu E,
+// out = F S L / rho^2.  F, L, out: D x D per point, column-major; L == nullptr means identity.
+template <int D, class T>
+__global__ __launch_bounds__(256) void k_tlsph_pk1_rho2(const T* __restrict__ F, const T* __restrict__ L,
+                                                        const T* __restrict__ rho, int64_t n, T lambda, T mu,
+                                                        T* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T f[D * D], E[D * D], S[D * D], P[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) f[k] = F[(int64_t)D * D * i + k];
+    T tr = (T)0;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            T v = (T)0;  // (F^T F)_rc = sum_k F_kr F_kc
+#pragma unroll
+            for (int k = 0; k < D; ++k) v = fma(f[k + D * r], f[k + D * c], v);
+            E[r + D * c] = (T)0.5 * (v - (r == c ? (T)1 : (T)0));
+        }
+#pragma unroll
+    for (int d = 0; d < D; ++d) tr += E[d + D * d];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) S[k] = (T)2 * mu * E[k];
+#pragma unroll
+    for (int d = 0; d < D; ++d) S[d + D * d] += lambda * tr;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            T v = (T)0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) v = fma(f[r + D * k], S[k + D * c], v);
+            P[r + D * c] = v;
+        }
+    const T ri = rho[i];
+    const T s = (T)1 / (ri * ri);
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            T v;
+            if (L) {
+                v = (T)0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) v = fma(P[r + D * k], L[(int64_t)D * D * i + k + D * c], v);
+            } else {
+                v = P[r + D * c];
+            }
+            out[(int64_t)D * D * i + r + D * c] = v * s;
+        }
+}
+
 }  // namespace pn

@@ -201,6 +201,99 @@ def tlsph_correction_matrix(nhs, X, volume, smoothing_length, kernel="wendland_c
     return tlsph_deformation_gradient(nhs, X, X, volume, smoothing_length, kernel, _invert=True)
 
 
+def _matrices(t, n, D, dtype, name):
+    """An (N, D, D) tensor -> its column-major per-point storage (a no-op for the views that
+    tlsph_deformation_gradient and tlsph_pk1_corrected return)."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise ArgumentError(f"{name} must be a GPU tensor")
+    if tuple(t.shape) != (n, D, D) or t.dtype != dtype:
+        raise ArgumentError(f"{name} must be an ({n}, {D}, {D}) tensor of {dtype}")
+    return t.transpose(1, 2).contiguous()
+
+
+def lame_parameters(young_modulus, poisson_ratio):
+    """(lambda, mu) of an isotropic linear-elastic material."""
+    E, nu = float(young_modulus), float(poisson_ratio)
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def tlsph_pk1_corrected(F, rho, young_modulus, poisson_ratio, correction=None):
+    """Corrected first Piola-Kirchhoff stress over rho^2 of a Saint Venant-Kirchhoff material
+    (TrixiParticles compute_pk1_corrected!, which the reference's benchmark_tlsph runs once at
+    benchmarks/smoothed_particle_hydrodynamics.jl:186):
+
+        E = (F^T F - I) / 2,  S = lambda tr(E) I + 2 mu E,  out = F S L / rho^2
+
+    F, correction (L, None: identity): (N, D, D); rho: material density per point.  Returns an
+    (N, D, D) tensor, one HIP kernel."""
+    import torch
+    from .util import dtype_code
+    if not isinstance(F, torch.Tensor) or F.dim() != 3:
+        raise ArgumentError("F must be an (N, D, D) GPU tensor")
+    n, D = F.shape[0], F.shape[1]
+    Fc = _matrices(F, n, D, F.dtype, "F")
+    Lc = None if correction is None else _matrices(correction, n, D, F.dtype, "correction")
+    _field(rho, n, F.dtype, "rho")
+    lam, mu = lame_parameters(young_modulus, poisson_ratio)
+    out = torch.empty((n, D, D), dtype=F.dtype, device=F.device)  # out[a] holds the matrix column-major
+    check(load().pn_tlsph_pk1_rho2(_vp(Fc), _vp(Lc) if Lc is not None else None, _vp(rho), n, D,
+                                   dtype_code(_device.np_dtype(F)), lam, mu, _vp(out),
+                                   ctypes.c_void_p(_device.current_stream_handle(F.device))), "tlsph_pk1_corrected")
+    return out.transpose(1, 2)
+
+
+def tlsph_interaction(nhs, X, x, mass, rho, pk1_rho2, smoothing_length, kernel="wendland_c2", F=None,
+                      young_modulus=0.0, penalty_alpha=0.0, dv=None, points=None):
+    """TLSPH interaction forces over the stored lists of `nhs`, a PrecomputedNeighborhoodSearch of
+    the initial coordinates X (TrixiParticles interact_structure_structure!, the loop the
+    reference times in benchmark_tlsph, benchmarks/smoothed_particle_hydrodynamics.jl:115-121):
+
+        dv_a += sum_b m_b (Pr_a + Pr_b) grad_a W(X_ab)                       (stress)
+              + (alpha/2) V_a V_b W / |X_ab|^2 E delta_ab x_ab / |x_ab| / m_a  (penalty)
+
+    with delta_ab = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab| (Ganzenmueller's hourglass
+    control, PenaltyForceGanzenmueller; on when penalty_alpha > 0, then F is required).
+    X, x: D x N; mass, rho: per point; pk1_rho2, F: (N, D, D).  dv: D x N with the coordinate
+    layout (None: a new zeroed one; given: accumulated into).  One HIP kernel."""
+    from .util import dtype_code
+    nl = getattr(nhs, "neighbor_lists", None)
+    if nl is None:
+        raise ArgumentError("tlsph_interaction needs a PrecomputedNeighborhoodSearch")
+    if nhs.periodic_box is not None:
+        raise ArgumentError("tlsph_interaction does not support a PeriodicBox")
+    D = nhs.ndims
+    X = _device.as_coordinates(X, D, "initial coordinates")
+    x = _device.as_coordinates(x, D, "current coordinates")
+    if X.dtype != x.dtype or X.shape != x.shape:
+        raise ArgumentError("X and x must have the same dtype and shape")
+    n = X.shape[1]
+    _field(mass, n, X.dtype, "mass")
+    _field(rho, n, X.dtype, "rho")
+    Pc = _matrices(pk1_rho2, n, D, X.dtype, "pk1_rho2")
+    alpha = float(penalty_alpha)
+    if not alpha >= 0.0:
+        raise ArgumentError("penalty_alpha must be non-negative")
+    Fc = None
+    if alpha > 0.0:
+        if F is None:
+            raise ArgumentError("the penalty force needs the deformation gradient F")
+        Fc = _matrices(F, n, D, X.dtype, "F")
+    if dv is None:
+        dv, acc = _vec_like(X), 0
+    else:
+        _field(dv, n, X.dtype, "dv", ndims=D)
+        acc = 1
+    first, count, ptr, keep = _device.selection(points, n, X.device)
+    check(load().pn_list_tlsph_interaction(
+        _vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, D, dtype_code(_device.np_dtype(X)), _kernel_code(kernel),
+        float(smoothing_length), _vp(X), _vp(x), _vp(mass), _vp(rho), _vp(Pc), _vp(Fc) if Fc is not None else None,
+        float(young_modulus), alpha, first, count, ctypes.c_void_p(ptr) if ptr else None, _vp(dv), acc,
+        ctypes.c_void_p(_device.current_stream_handle(X.device))), "tlsph_interaction")
+    del keep
+    return dv
+
+
 def tait_pressure_term(rho, rho0, c, gamma=7.0, out=None):
     """P = p / rho^2 with p = tait_eos(rho, ...): the per-point term that
     SPHPressureAcceleration takes, in one HIP kernel."""

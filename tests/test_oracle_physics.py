@@ -174,3 +174,111 @@ def test_tlsph_grad_translation_and_locality(orc):
     rows = {a for a in range(len(X)) if k in cols[rp[a]:rp[a + 1]]}
     changed = {a for a in range(len(X)) if not np.allclose(F2[a], F[a], rtol=0, atol=1e-14)}
     assert changed <= rows and k in changed
+
+
+# ---- TLSPH interaction (TrixiParticles interact_structure_structure!) ------------------------
+
+def _svk_energy(F, V, lam, mu):
+    """sum_c V_c Psi(F_c), Psi = mu E:E + lambda/2 tr(E)^2, E = (F^T F - I)/2 (numpy)."""
+    nd = F.shape[1]
+    E = 0.5 * (np.einsum("nki,nkj->nij", F, F) - np.eye(nd))
+    tr = np.trace(E, axis1=1, axis2=2)
+    return float(np.sum(V * (mu * np.sum(E * E, axis=(1, 2)) + 0.5 * lam * tr * tr)))
+
+
+@pytest.mark.parametrize("kernel", ["wendland_c2", "cubic_spline"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_tlsph_stress_force_is_energy_gradient(orc, kernel, nd):
+    """With one material density and a symmetric correction L, the stress force is exactly the
+    negative gradient of the discrete strain energy: m_a dv_a = -dE/dx_a, E = sum_c V_c Psi(F_c),
+    F from orc_tlsph_grad, Psi the Saint Venant-Kirchhoff energy (numpy), checked by central
+    differences.  L applied on the wrong side or left out, the second Piola-Kirchhoff stress in
+    place of the first, a wrong sign or a wrong rho power fails it."""
+    shape = (6, 5, 5) if nd == 3 else (9, 8)
+    X = synthetic.jittered_lattice(shape, 1.0, 0.15, dtype=np.float64, seed=11)
+    h = 1.2
+    lists = neighbor_lists(orc, X, 2 * h)
+    n = len(X)
+    rho = np.full(n, 1000.0)
+    V = 1.0 + 0.2 * synthetic.uniform(8, n)
+    m = V * rho
+    M = orc.tlsph_grad(X, X, lists, V, h, kernel)
+    L = np.linalg.inv(M)
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    lam, mu = 0.7, 0.45
+    x = X + 0.06 * np.sin(1.3 * X[:, ::-1]) + 0.02 * X
+    F = orc.tlsph_grad(X, x, lists, V, h, kernel, L=L)
+    Pr = orc.tlsph_pk1_rho2(F, rho, lam, mu, L=L)
+    dv = orc.tlsph_force(X, x, lists, m, rho, Pr, h, kernel)
+    energy = lambda y: _svk_energy(orc.tlsph_grad(X, y, lists, V, h, kernel, L=L), V, lam, mu)
+    eps = 1e-6
+    scale = np.abs(m[:, None] * dv).max()
+    for a in [0, n // 3, n // 2, n - 1]:
+        for d in range(nd):
+            xp, xm = x.copy(), x.copy()
+            xp[a, d] += eps
+            xm[a, d] -= eps
+            g = (energy(xp) - energy(xm)) / (2 * eps)
+            assert -m[a] * dv[a, d] == pytest.approx(g, rel=1e-5, abs=1e-6 * scale)
+
+
+def _two_points(nd, d=1.0, squeeze=0.0):
+    X = np.zeros((2, nd))
+    X[1, 0] = d
+    x = X.copy()
+    x[1, 0] = d - squeeze
+    lists = (np.array([0, 2, 4]), np.array([0, 1, 0, 1], dtype=np.int32))
+    return X, x, lists
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_tlsph_pair_signs_and_penalty_value(orc, nd):
+    """Tension (Pr = p I, p > 0) pulls a pair together, compression pushes it apart; a pair
+    squeezed against F = I is pushed apart by the penalty with Ganzenmueller's magnitude
+    (alpha/2) V_a V_b W(|X_ab|) / |X_ab|^2 E delta / m_a, here delta = 2 * squeeze."""
+    h = 0.8
+    X, x, lists = _two_points(nd, 1.0, 0.1)
+    m, rho = np.array([2.0, 3.0]), np.array([4.0, 5.0])
+    eye = np.broadcast_to(np.eye(nd), (2, nd, nd))
+    for p, sign in [(1.0, +1), (-1.0, -1)]:
+        dv = orc.tlsph_force(X, X, lists, m, rho, p * eye, h)
+        assert sign * dv[0, 0] > 0 and sign * dv[1, 0] < 0
+        assert m[0] * dv[0] == pytest.approx(-m[1] * dv[1], rel=1e-14)
+    zero = np.zeros((2, nd, nd))
+    alpha, E = 0.1, 1.4e6
+    dv = orc.tlsph_force(X, x, lists, m, rho, zero, h, F=eye, young_modulus=E, penalty_alpha=alpha)
+    Va, Vb = m / rho
+    # W(|X_ab| = 1) of Wendland C2 through the density sum rho_0 = 0 W(0) + 1 W(1)
+    w1 = orc.sph_density(X, lists, np.array([0.0, 1.0]), h, "wendland_c2")[0][0]
+    assert w1 > 0
+    want = 0.5 * alpha * Va * Vb * w1 / 1.0**2 * E * (2 * 0.1) / m[0]
+    assert dv[0, 0] == pytest.approx(-want, rel=1e-12)
+    assert np.all(dv[0, 1:] == 0) and m[0] * dv[0, 0] == pytest.approx(-m[1] * dv[1, 0], rel=1e-12)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_tlsph_momentum_and_affine_penalty(orc, nd):
+    """Every pair term is antisymmetric: sum_a m_a dv_a = 0 with arbitrary stresses and
+    deformation gradients, penalty on.  Under an affine map x = A X + c with F_a = A the
+    penalty adds exactly nothing (its error vector vanishes)."""
+    shape = (7, 6, 5) if nd == 3 else (12, 11)
+    X = synthetic.jittered_lattice(shape, 1.0, 0.2, dtype=np.float64, seed=5)
+    h = 1.1
+    lists = neighbor_lists(orc, X, 2 * h)
+    n = len(X)
+    rng = np.random.default_rng(2)
+    m = 1.0 + 0.3 * rng.random(n)
+    rho = 1.0 + 0.3 * rng.random(n)
+    Pr = rng.standard_normal((n, nd, nd))
+    F = np.eye(nd) + 0.2 * rng.standard_normal((n, nd, nd))
+    x = X + 0.1 * rng.standard_normal(X.shape)
+    dv = orc.tlsph_force(X, x, lists, m, rho, Pr, h, F=F, young_modulus=3.0, penalty_alpha=0.1)
+    scale = np.abs(m[:, None] * dv).max()
+    assert np.abs((m[:, None] * dv).sum(axis=0)).max() <= 1e-12 * n * scale
+    A = np.eye(nd) + 0.25 * rng.standard_normal((nd, nd))
+    xa = X @ A.T + 0.7
+    FA = np.broadcast_to(A, (n, nd, nd))
+    base = orc.tlsph_force(X, xa, lists, m, rho, Pr, h)
+    pen = orc.tlsph_force(X, xa, lists, m, rho, Pr, h, F=FA, young_modulus=3.0, penalty_alpha=0.1)
+    assert np.allclose(pen, base, rtol=0, atol=1e-12 * np.abs(base).max())
+    assert not np.allclose(dv, orc.tlsph_force(X, x, lists, m, rho, Pr, h), rtol=0, atol=1e-6)

@@ -303,13 +303,23 @@ int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_id
  * (the penalty force of Ganzenmueller 2015, PenaltyForceGanzenmueller; skipped when
  * penalty_alpha == 0, then x, rho and F may be NULL).  X (initial), x (current): D x N;
  * mass and rho (material density) per point; pk1_rho2 = pk1_corrected / rho^2 and F: D x D per
- * point, column-major.  dv: D x N.  One floating type for all arrays.  Enqueue only. */
+ * point, column-major.  dv: D x N.  The lists are X's own (n_rows = N, columns index X).  One
+ * floating type for all arrays.  workspace: NULL, or
+ * pn_tlsph_interaction_workspace_bytes(ndims, dtype, n_rows) bytes of 16-byte aligned device
+ * memory; with the penalty on, the call then packs each point's fields into one record there
+ * and gathers those (same results bit for bit, fewer loads).  Enqueue only. */
 int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows,
                               int stored_base, int ndims, int dtype, int kernel, double h,
                               const void* X, const void* x, const void* mass, const void* rho,
                               const void* pk1_rho2, const void* F, double young_modulus,
                               double penalty_alpha, int64_t sel_first, int64_t sel_count,
-                              const int32_t* sel_list, void* dv, int accumulate, void* stream);
+                              const int32_t* sel_list, void* dv, int accumulate, void* workspace,
+                              void* stream);
+
+/* Size in bytes of pn_list_tlsph_interaction's workspace (-1 for an unsupported ndims/dtype):
+ * n_rows records of 2 (D + 1) + 2 D^2 values rounded up to 16 bytes.  No counterpart in the
+ * reference (an implementation buffer). */
+int64_t pn_tlsph_interaction_workspace_bytes(int ndims, int dtype, int64_t n_rows);
 
 /* Corrected first Piola-Kirchhoff stress over rho^2 of a Saint Venant-Kirchhoff material
  * (TrixiParticles compute_pk1_corrected!, run once by the set-up of benchmark_tlsph,

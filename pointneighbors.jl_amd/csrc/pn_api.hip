@@ -443,7 +443,7 @@ int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, in
                               int ndims, int dtype, int kernel, double h, const void* X, const void* x,
                               const void* mass, const void* rho, const void* pk1_rho2, const void* F,
                               double young_modulus, double penalty_alpha, int64_t sel_first, int64_t sel_count,
-                              const int32_t* sel_list, void* dv, int accumulate, void* stream) {
+                              const int32_t* sel_list, void* dv, int accumulate, void* workspace, void* stream) {
     if (!row_ptr || !X || !mass || !pk1_rho2 || !dv) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     if (ndims != 2 && ndims != 3) return fail(PN_ERR_UNSUPPORTED, "the TLSPH kernels are defined for 2D and 3D");
     if (kernel != pn::KERNEL_CUBIC_SPLINE && kernel != pn::KERNEL_WENDLAND_C2)
@@ -463,7 +463,16 @@ int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, in
         using K = pn::SmoothingKernel<T, D>;
         auto kern = K::make(kernel, h);
         const T hae = (T)(0.5 * penalty_alpha * young_modulus);
-        if (pen)
+        if (pen && workspace) {
+            uint4* rec = (uint4*)workspace;
+            hipLaunchKernelGGL((pn::k_tlsph_pack<D, T>), dim3(blocks_for(n_rows, 256)), dim3(256), 0,
+                               (hipStream_t)stream, (const T*)X, (const T*)x, (const T*)mass, (const T*)rho,
+                               (const T*)pk1_rho2, (const T*)F, n_rows, rec);
+            PN_CHECK_LAUNCH();
+            hipLaunchKernelGGL((pn::k_list_tlsph_force_rec<D, T, K>), dim3(nblocks), dim3(256), 0, (hipStream_t)stream,
+                               row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count, rec, (const T*)rho, hae,
+                               kern, (T*)dv, accumulate ? 1 : 0, nblocks);
+        } else if (pen)
             hipLaunchKernelGGL((pn::k_list_tlsph_force<D, T, K, true>), dim3(nblocks), dim3(256), 0,
                                (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
                                (const T*)X, (const T*)x, (const T*)mass, (const T*)rho, (const T*)pk1_rho2,
@@ -478,6 +487,16 @@ int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, in
     };
     if (ndims == 2) return dtype == PN_FLOAT32 ? run(pn::IntC<2>{}, 0.0f) : run(pn::IntC<2>{}, 0.0);
     return dtype == PN_FLOAT32 ? run(pn::IntC<3>{}, 0.0f) : run(pn::IntC<3>{}, 0.0);
+}
+
+int64_t pn_tlsph_interaction_workspace_bytes(int ndims, int dtype, int64_t n_rows) {
+    if ((ndims != 2 && ndims != 3) || (dtype != PN_FLOAT32 && dtype != PN_FLOAT64) || n_rows < 0) return -1;
+    auto per = [&](auto dc, auto tv) -> int64_t {
+        return 16 * (int64_t)pn::TlsphRec<decltype(dc)::value, decltype(tv)>::NV;
+    };
+    const int64_t b = ndims == 2 ? (dtype == PN_FLOAT32 ? per(pn::IntC<2>{}, 0.0f) : per(pn::IntC<2>{}, 0.0))
+                                 : (dtype == PN_FLOAT32 ? per(pn::IntC<3>{}, 0.0f) : per(pn::IntC<3>{}, 0.0));
+    return b * n_rows;
 }
 
 int pn_tlsph_pk1_rho2(const void* F, const void* L, const void* rho, int64_t n, int ndims, int dtype,

@@ -257,7 +257,7 @@ struct ListPeriodic {
 // order), mass per y point, rho per x point.
 constexpr int LIST_LANES = 16;
 constexpr int LIST_ROWS_PER_BLOCK = 256 / LIST_LANES;
-constexpr int LIST_UNROLL = 8;
+constexpr int LIST_UNROLL = 4;  // measured best of 1, 2, 4, 8 for the density sum
 
 template <int D, class TC, class TR, class K>
 __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict__ row_ptr,
@@ -361,16 +361,17 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
 #pragma unroll
         for (int c = 0; c < D; ++c) g[r][c] = (T)0;
     }
-    for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * LIST_UNROLL) {
-        int64_t j[LIST_UNROLL];
+    constexpr int UG = 2;  // gathers in flight per lane: 2 measured 1.7x faster than 8 (occupancy)
+    for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * UG) {
+        int64_t j[UG];
 #pragma unroll
-        for (int u = 0; u < LIST_UNROLL; ++u) {
+        for (int u = 0; u < UG; ++u) {
             const int k = k0 + LIST_LANES * u;
             j[u] = k < e ? (int64_t)col[k] - stored_base : -1;
         }
-        T Xj[LIST_UNROLL][D], xj[LIST_UNROLL][D], vj[LIST_UNROLL];
+        T Xj[UG][D], xj[UG][D], vj[UG];
 #pragma unroll
-        for (int u = 0; u < LIST_UNROLL; ++u) {
+        for (int u = 0; u < UG; ++u) {
             const int64_t jj = j[u] >= 0 ? j[u] : 0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
             vj[u] = vol[jj];
         }
 #pragma unroll
-        for (int u = 0; u < LIST_UNROLL; ++u) {
+        for (int u = 0; u < UG; ++u) {
             T Xab[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) Xab[d] = Xi[d] - Xj[u][d];
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restr
     T acc[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = (T)0;
-    constexpr int U = PEN ? 2 : 4;  // a neighbour brings D*D + D + 1 values, 2 D*D + 2 D + 2 with PEN
+    constexpr int U = 2;  // gathers in flight per lane (measured best: deeper unrolls cost occupancy)
     for (int k0 = b + sub; k0 < e; k0 += LIST_LANES * U) {
         int64_t j[U];
 #pragma unroll
@@ -571,6 +572,134 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restr
                 for (int d = 0; d < D; ++d) acc[d] = fma(g, xab[d], acc[d]);
             }
         }
+    }
+#pragma unroll
+    for (int o = LIST_LANES / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], o, LIST_LANES);
+    }
+    if (sub != 0) return;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        T* o = dv + (int64_t)D * i + d;
+        *o = accumulate ? *o + acc[d] : acc[d];
+    }
+}
+
+// Per-point record of the penalty interaction, one 16-byte aligned block per point so that a
+// neighbour costs TlsphRec::NV vector loads instead of 2 D^2 + 2 D + 2 scalar ones:
+//     [X (D), m, x (D), V = m / rho, Pr (D*D, column-major), F (D*D, column-major), pad]
+template <int D, class T>
+struct TlsphRec {
+    static constexpr int USED = 2 * (D + 1) + 2 * D * D;
+    static constexpr int PER16 = 16 / (int)sizeof(T);
+    static constexpr int R = (USED + PER16 - 1) / PER16 * PER16;  // values per point
+    static constexpr int NV = R / PER16;                          // 16-byte loads per point
+    static constexpr int X = 0, M = D, XC = D + 1, V = 2 * D + 1, P = 2 * D + 2, F = 2 * D + 2 + D * D;
+};
+
+template <int D, class T>
+__global__ __launch_bounds__(256) void k_tlsph_pack(const T* __restrict__ X, const T* __restrict__ x,
+                                                    const T* __restrict__ mass, const T* __restrict__ rho,
+                                                    const T* __restrict__ Pr, const T* __restrict__ F, int64_t n,
+                                                    uint4* __restrict__ rec) {
+    using RC = TlsphRec<D, T>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T r[RC::R];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        r[RC::X + d] = X[(int64_t)D * i + d];
+        r[RC::XC + d] = x[(int64_t)D * i + d];
+    }
+    r[RC::M] = mass[i];
+    r[RC::V] = mass[i] / rho[i];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) {
+        r[RC::P + k] = Pr[(int64_t)D * D * i + k];
+        r[RC::F + k] = F[(int64_t)D * D * i + k];
+    }
+#pragma unroll
+    for (int k = RC::USED; k < RC::R; ++k) r[k] = (T)0;
+#pragma unroll
+    for (int q = 0; q < RC::NV; ++q) {
+        uint4 v;
+        __builtin_memcpy(&v, &r[q * RC::PER16], 16);
+        rec[(int64_t)RC::NV * i + q] = v;
+    }
+}
+
+template <int D, class T>
+__device__ __forceinline__ void load_rec(const uint4* __restrict__ rec, int64_t i, T (&r)[TlsphRec<D, T>::R]) {
+    using RC = TlsphRec<D, T>;
+#pragma unroll
+    for (int q = 0; q < RC::NV; ++q) {
+        const uint4 v = rec[(int64_t)RC::NV * i + q];
+        __builtin_memcpy(&r[q * RC::PER16], &v, 16);
+    }
+}
+
+// k_list_tlsph_force<D, T, K, true> over packed records (same terms, same operation order, so
+// the same bits); one neighbour in flight per lane, which measured fastest with the wide loads.
+template <int D, class T, class K>
+__global__ __launch_bounds__(256) void k_list_tlsph_force_rec(const int32_t* __restrict__ row_ptr,
+                                                              const int32_t* __restrict__ col, int stored_base,
+                                                              int64_t sel_first, const int32_t* __restrict__ sel_list,
+                                                              int64_t n_sel, const uint4* __restrict__ rec,
+                                                              const T* __restrict__ rho, T half_alpha_E, K kern,
+                                                              T* __restrict__ dv, int accumulate, int nblocks) {
+    using RC = TlsphRec<D, T>;
+    const int sub = threadIdx.x & (LIST_LANES - 1);
+    const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
+    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    const int64_t i = sel_point(a, sel_first, sel_list);
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    T ri[RC::R];
+    load_rec<D, T>(rec, i, ri);
+    const T pen_a = half_alpha_E / rho[i];
+    const T tiny = (T)1.4901161193847656e-8;  // sqrt(eps(Float64)): coincident points add nothing
+    T acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = (T)0;
+    for (int k = b + sub; k < e; k += LIST_LANES) {
+        T rj[RC::R];
+        load_rec<D, T>(rec, (int64_t)col[k] - stored_base, rj);
+        T Xab[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) Xab[d] = ri[RC::X + d] - rj[RC::X + d];
+        T d2 = Xab[0] * Xab[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
+        const T dist = sqrt(d2);
+        const bool use = dist >= tiny;
+        const T safe = use ? dist : (T)1;
+        const T s = use ? rj[RC::M] * kern.dw(safe) / safe : (T)0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            T v = (T)0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) v = fma(ri[RC::P + r + D * c] + rj[RC::P + r + D * c], Xab[c], v);
+            acc[r] = fma(s, v, acc[r]);
+        }
+        T xab[D], eps[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) xab[d] = ri[RC::XC + d] - rj[RC::XC + d];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            T v = (T)-2 * xab[r];
+#pragma unroll
+            for (int c = 0; c < D; ++c) v = fma(ri[RC::F + r + D * c] + rj[RC::F + r + D * c], Xab[c], v);
+            eps[r] = v;
+        }
+        T c2 = xab[0] * xab[0], dl = eps[0] * xab[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+            c2 = c2 + xab[d] * xab[d];
+            dl = fma(eps[d], xab[d], dl);
+        }
+        const T g = use ? pen_a * rj[RC::V] * kern.w(safe) / (safe * safe) * dl / c2 : (T)0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] = fma(g, xab[d], acc[d]);
     }
 #pragma unroll
     for (int o = LIST_LANES / 2; o > 0; o >>= 1) {

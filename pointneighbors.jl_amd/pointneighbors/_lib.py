@@ -93,7 +93,9 @@ SIGNATURES = {
     "pn_list_tlsph_deformation_grad": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp,
                                                 _vp, _vp, _c_int, _c_i64, _c_i64, _vp, _vp, _vp]),
     "pn_list_tlsph_interaction": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp,
-                                           _vp, _vp, _vp, _c_dbl, _c_dbl, _c_i64, _c_i64, _vp, _vp, _c_int, _vp]),
+                                           _vp, _vp, _vp, _c_dbl, _c_dbl, _c_i64, _c_i64, _vp, _vp, _c_int, _vp,
+                                           _vp]),
+    "pn_tlsph_interaction_workspace_bytes": (_c_i64, [_c_int, _c_int, _c_i64]),
     "pn_tlsph_pk1_rho2": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp]),
     "pn_list_sph_density_periodic": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                               _vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _c_int, _c_dbl, _p_dbl,

@@ -177,6 +177,8 @@ def tlsph_deformation_gradient(nhs, X, x, volume, smoothing_length, kernel="wend
     if X.dtype != x.dtype or X.shape != x.shape:
         raise ArgumentError("X and x must have the same dtype and shape")
     n = X.shape[1]
+    if nl.n != n:
+        raise ArgumentError(f"the neighbor lists have {nl.n} rows for {n} points: build them on X")
     _field(volume, n, X.dtype, "volume")
     out = torch.empty((n, D, D), dtype=X.dtype, device=X.device)  # out[a] holds F_a column-major
     L = None
@@ -244,7 +246,7 @@ def tlsph_pk1_corrected(F, rho, young_modulus, poisson_ratio, correction=None):
 
 
 def tlsph_interaction(nhs, X, x, mass, rho, pk1_rho2, smoothing_length, kernel="wendland_c2", F=None,
-                      young_modulus=0.0, penalty_alpha=0.0, dv=None, points=None):
+                      young_modulus=0.0, penalty_alpha=0.0, dv=None, points=None, _packed=True):
     """TLSPH interaction forces over the stored lists of `nhs`, a PrecomputedNeighborhoodSearch of
     the initial coordinates X (TrixiParticles interact_structure_structure!, the loop the
     reference times in benchmark_tlsph, benchmarks/smoothed_particle_hydrodynamics.jl:115-121):
@@ -255,7 +257,10 @@ def tlsph_interaction(nhs, X, x, mass, rho, pk1_rho2, smoothing_length, kernel="
     with delta_ab = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab| (Ganzenmueller's hourglass
     control, PenaltyForceGanzenmueller; on when penalty_alpha > 0, then F is required).
     X, x: D x N; mass, rho: per point; pk1_rho2, F: (N, D, D).  dv: D x N with the coordinate
-    layout (None: a new zeroed one; given: accumulated into).  One HIP kernel."""
+    layout (None: a new zeroed one; given: accumulated into).  One HIP kernel; with the penalty
+    on, a pack kernel first writes each point's fields into one record of a workspace (fewer,
+    wider gathers; the same bits)."""
+    import torch
     from .util import dtype_code
     nl = getattr(nhs, "neighbor_lists", None)
     if nl is None:
@@ -268,6 +273,8 @@ def tlsph_interaction(nhs, X, x, mass, rho, pk1_rho2, smoothing_length, kernel="
     if X.dtype != x.dtype or X.shape != x.shape:
         raise ArgumentError("X and x must have the same dtype and shape")
     n = X.shape[1]
+    if nl.n != n:
+        raise ArgumentError(f"the neighbor lists have {nl.n} rows for {n} points: build them on X")
     _field(mass, n, X.dtype, "mass")
     _field(rho, n, X.dtype, "rho")
     Pc = _matrices(pk1_rho2, n, D, X.dtype, "pk1_rho2")
@@ -284,12 +291,17 @@ def tlsph_interaction(nhs, X, x, mass, rho, pk1_rho2, smoothing_length, kernel="
     else:
         _field(dv, n, X.dtype, "dv", ndims=D)
         acc = 1
+    ws = None
+    if Fc is not None and _packed:
+        nbytes = load().pn_tlsph_interaction_workspace_bytes(D, dtype_code(_device.np_dtype(X)), nl.n)
+        ws = torch.empty(((nbytes + 15) // 16, 4), dtype=torch.int32, device=X.device)
     first, count, ptr, keep = _device.selection(points, n, X.device)
     check(load().pn_list_tlsph_interaction(
         _vp(nl.row_ptr), _vp(nl.col_idx), nl.n, 0, D, dtype_code(_device.np_dtype(X)), _kernel_code(kernel),
         float(smoothing_length), _vp(X), _vp(x), _vp(mass), _vp(rho), _vp(Pc), _vp(Fc) if Fc is not None else None,
         float(young_modulus), alpha, first, count, ctypes.c_void_p(ptr) if ptr else None, _vp(dv), acc,
-        ctypes.c_void_p(_device.current_stream_handle(X.device))), "tlsph_interaction")
+        _vp(ws) if ws is not None else None, ctypes.c_void_p(_device.current_stream_handle(X.device))),
+        "tlsph_interaction")
     del keep
     return dv
 

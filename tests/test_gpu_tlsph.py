@@ -183,6 +183,30 @@ def test_interaction_momentum_rows_and_accumulate():
     assert torch.equal(sph.tlsph_interaction(p, Xd, x, md, rd, Pr, h, **kw), dv)
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(10, 9, 8), (25, 24)])
+def test_packed_records_same_bits(shape, dt):
+    """The penalty interaction over packed per-point records equals the field-by-field gathers
+    bit for bit, also for a row subset and accumulation."""
+    h = 1.2
+    X, Xd, p, _, _ = setup(shape, dt, h, seed=8)
+    n, nd = X.shape
+    rng = np.random.default_rng(5)
+    x = dev((X + 0.1 * rng.standard_normal(X.shape)).astype(dt))
+    m, rho = materials(n, dt)
+    Pr = cuda((1e-3 * rng.standard_normal((n, nd, nd))).astype(dt))
+    F = cuda((np.eye(nd) + 0.2 * rng.standard_normal((n, nd, nd))).astype(dt))
+    kw = dict(F=F, young_modulus=1e4, penalty_alpha=0.1)
+    args = (p, Xd, x, cuda(m), cuda(rho), Pr, h)
+    a = sph.tlsph_interaction(*args, **kw)
+    b = sph.tlsph_interaction(*args, _packed=False, **kw)
+    assert torch.equal(a, b)
+    base = torch.randn_like(a.t()).t()
+    sa = sph.tlsph_interaction(*args, points=[3, 1, 200, 7], dv=base.clone(), **kw)
+    sb = sph.tlsph_interaction(*args, points=[3, 1, 200, 7], dv=base.clone(), _packed=False, **kw)
+    assert torch.equal(sa, sb)
+
+
 def test_interaction_argument_errors():
     h = 1.2
     X, Xd, p, _, _ = setup((5, 5, 5), np.float32, h)
@@ -195,3 +219,5 @@ def test_interaction_argument_errors():
         sph.tlsph_interaction(p, Xd, Xd, cuda(m), cuda(rho), Pr[:, :2], h)
     with pytest.raises(pn.ArgumentError):
         sph.tlsph_interaction(p, Xd, Xd, cuda(m), cuda(rho), Pr.double(), h)
+    with pytest.raises(pn.ArgumentError):  # lists of another cloud
+        sph.tlsph_interaction(p, Xd[:, :-1], Xd[:, :-1], cuda(m[:-1]), cuda(rho[:-1]), Pr[:-1], h)

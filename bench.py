@@ -477,7 +477,8 @@ def bench_tlsph(args, dev):
     E 1.4e6, nu 0.4, Ganzenmueller penalty alpha 0.1), Float32, precomputed lists of the initial
     coordinates.  Times the two loops the reference times: interact_structure_structure!
     (benchmark_tlsph, accumulating into dv) and calc_deformation_grad!
-    (benchmark_tlsph_deformation_grad), both per call with the lists resident."""
+    (benchmark_tlsph_deformation_grad), both per call with the lists resident.  A Float64 pass
+    of the same cloud is reported beside the Float32 headline."""
     import torch
     import pointneighbors as pn
     from pointneighbors import sph, synthetic
@@ -491,25 +492,34 @@ def bench_tlsph(args, dev):
     X = X[np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))].astype(np.float32)
     n = len(X)
     h = float(r) / 2
-    pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=r, n_points=n)
-    Xd = synthetic.to_device_coords(X, dev)
-    pn.initialize(pre, Xd, Xd)
-    nnz = int(pre.neighbor_lists.col_idx.numel())
     rho0, m0, E, nu, alpha = 1000.0, 0.1, 1.4e6, 0.4, 0.1
-    mass = torch.full((n,), m0, dtype=torch.float32, device=dev)
-    rho = torch.full((n,), rho0, dtype=torch.float32, device=dev)
-    vol = mass / rho
     k = max(2, min(args.steps, 10))
-    corr_ms = _timed(lambda: sph.tlsph_correction_matrix(pre, Xd, vol, h), k)
-    L = sph.tlsph_correction_matrix(pre, Xd, vol, h)
-    x = Xd * 1.01 + 0.002 * torch.sin(Xd)
-    grad_ms = _timed(lambda: sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L), k)
-    F = sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L)
-    pk1_ms = _timed(lambda: sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L), k)
-    Pr = sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L)
-    dv = sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E, penalty_alpha=alpha)
-    inter_ms = _timed(lambda: sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E,
-                                                    penalty_alpha=alpha, dv=dv), k)
+
+    def run(pts, radius, tdt):
+        pre = pn.PrecomputedNeighborhoodSearch[3](search_radius=radius, n_points=n)
+        Xd = synthetic.to_device_coords(pts, dev)
+        pn.initialize(pre, Xd, Xd)
+        mass = torch.full((n,), m0, dtype=tdt, device=dev)
+        rho = torch.full((n,), rho0, dtype=tdt, device=dev)
+        vol = mass / rho
+        res = {"nnz": int(pre.neighbor_lists.col_idx.numel())}
+        res["correction_matrix_ms"] = _timed(lambda: sph.tlsph_correction_matrix(pre, Xd, vol, h), k)
+        L = sph.tlsph_correction_matrix(pre, Xd, vol, h)
+        x = Xd * 1.01 + 0.002 * torch.sin(Xd)
+        res["deformation_grad_ms"] = _timed(lambda: sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L), k)
+        F = sph.tlsph_deformation_gradient(pre, Xd, x, vol, h, correction=L)
+        res["pk1_corrected_ms"] = _timed(lambda: sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L), k)
+        Pr = sph.tlsph_pk1_corrected(F, rho, E, nu, correction=L)
+        dv = sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E, penalty_alpha=alpha)
+        res["interaction_ms"] = _timed(lambda: sph.tlsph_interaction(pre, Xd, x, mass, rho, Pr, h, F=F, young_modulus=E,
+                                                                     penalty_alpha=alpha, dv=dv), k)
+        res["finite"] = bool(torch.isfinite(F).all().item() and torch.isfinite(dv).all().item())
+        del pre
+        return res
+
+    r32 = run(X, r, torch.float32)
+    r64 = run(X.astype(np.float64), np.float64(r), torch.float64)
+    inter_ms, grad_ms, nnz = r32["interaction_ms"], r32["deformation_grad_ms"], r32["nnz"]
     h100_interact_ms, h100_grad_ms = 3.801, 0.825  # benchmarks/plot_benchmarks.jl:76,84
     return {
         "metric": "TLSPH interaction time (benchmark_tlsph, 1,030,301 points)", "value": inter_ms, "unit": "ms",
@@ -522,8 +532,10 @@ def bench_tlsph(args, dev):
         "nnz": nnz, "interaction_ms": inter_ms, "interaction_pairs_per_s": nnz / (inter_ms * 1e-3),
         "deformation_grad_ms": grad_ms, "deformation_grad_pairs_per_s": nnz / (grad_ms * 1e-3),
         "speedup_vs_h100": {"interaction": h100_interact_ms / inter_ms, "deformation_grad": h100_grad_ms / grad_ms},
-        "correction_matrix_ms": corr_ms, "pk1_corrected_ms": pk1_ms,
-        "finite": bool(torch.isfinite(F).all().item() and torch.isfinite(dv).all().item()),
+        "correction_matrix_ms": r32["correction_matrix_ms"], "pk1_corrected_ms": r32["pk1_corrected_ms"],
+        "finite": r32["finite"],
+        "f64": dict(r64, speedup_vs_h100={"interaction": 6.187 / r64["interaction_ms"],  # plot_benchmarks.jl:75,83
+                                          "deformation_grad": 1.423 / r64["deformation_grad_ms"]}),
     }
 
 

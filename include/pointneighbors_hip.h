@@ -80,6 +80,19 @@ int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
                    int32_t max_points_per_cell, pn_grid_t* out);
 int pn_grid_destroy(pn_grid_t grid);
 
+/* GridNeighborhoodSearch with a SpatialHashingCellList (src/cell_lists/spatial_hashing.jl:1-183;
+ * its traversal with the collision checks, src/nhs_grid.jl:479-575): an unbounded domain.  A
+ * point's cell is floor_to_int.(x ./ search_radius); its bucket is the reference's spatial_hash
+ * (Ihmsen et al. 2011) mod list_size.  pn_grid_update counting-sorts the points by bucket
+ * (max_points_per_cell bounds a bucket, as the reference's DynamicVectorOfVectors does:
+ * PN_ERR_CELL_FULL), and a cell coordinate outside Int32 (NaN, Inf included) is
+ * PN_ERR_OUT_OF_BOUNDS, the reference's InexactError.  Every traversal entry point works on the
+ * handle; a traversal scans the 3^D neighbouring cells' buckets and keeps a candidate only if
+ * its own cell is the scanned cell, so neighbour sets equal those of a FullGridCellList.  The
+ * cell type is the promotion of the coordinate and radius types.  No periodic box, no batching. */
+int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, double search_radius,
+                                int64_t list_size, int32_t max_points_per_cell, pn_grid_t* out);
+
 /* GridNeighborhoodSearch with a PeriodicBox (src/nhs_grid.jl:102-125, PeriodicBox
  * src/neighborhood_search.jl:129-141).  The caller sizes the periodic grid as the
  * reference constructor does:

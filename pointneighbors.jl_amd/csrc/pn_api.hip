@@ -57,10 +57,46 @@ int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
     return PN_OK;
 }
 
+int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, double search_radius,
+                                int64_t list_size, int32_t max_points_per_cell, pn_grid_t* out) {
+    if (!out) return fail(PN_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (ndims < 1 || ndims > 3) return fail(PN_ERR_ARGUMENT, "ndims must be 1, 2 or 3");
+    for (int t : {coord_dtype, radius_dtype})
+        if (t != PN_FLOAT32 && t != PN_FLOAT64) return fail(PN_ERR_ARGUMENT, "dtype must be PN_FLOAT32 or PN_FLOAT64");
+    if (!(search_radius > 0.0) || !std::isfinite(search_radius))
+        return fail(PN_ERR_ARGUMENT, "search_radius must be positive and finite");
+    if (max_points_per_cell < 1) return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be positive");
+    if (list_size < 1 || list_size > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "list_size must be in 1:2^31-2");
+    pn_grid* g = new pn_grid();
+    g->nd = ndims;
+    g->tc = coord_dtype;
+    g->tr = radius_dtype;
+    g->tcell = (coord_dtype == PN_FLOAT64 || radius_dtype == PN_FLOAT64) ? PN_FLOAT64 : PN_FLOAT32;
+    g->radius = search_radius;
+    g->max_ppc = max_points_per_cell;
+    g->geom.nd = ndims;
+    for (int d = 0; d < 3; ++d) {
+        g->geom.n[d] = 1;
+        g->geom.minc[d] = 0.0;
+        g->geom.csd[d] = search_radius;
+        g->geom.per[d] = 1;
+        g->geom.box[d] = 0.0;
+    }
+    g->geom.ncells = (int)list_size;
+    g->geom.cs = search_radius;
+    g->geom.periodic = 0;
+    g->geom.hash = 1;
+    g->geom.list_size = (int)list_size;
+    *out = g;
+    return PN_OK;
+}
+
 int pn_grid_set_periodic(pn_grid_t grid, const double* cell_size, const int64_t* n_periodic,
                          const double* box_size) {
     if (!grid || !cell_size || !n_periodic || !box_size) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     pn_grid& g = *grid;
+    if (g.geom.hash) return fail(PN_ERR_UNSUPPORTED, "periodic boxes are not supported with a SpatialHashingCellList");
     if (g.geom.table) return fail(PN_ERR_UNSUPPORTED, "periodic boxes are not supported for batched clouds");
     for (int d = 0; d < g.nd; ++d) {
         if (n_periodic[d] < 3)

@@ -41,6 +41,8 @@ struct Geom {
     const int32_t* cloud_of_point;
     const int32_t* cell_offsets;  // n_clouds + 1
     int n_clouds;
+    int hash;       // SpatialHashingCellList: keys are hash buckets of unbounded cells
+    int list_size;  // its number of buckets (== ncells)
 };
 
 // Cloud of a linear cell index in a batch (binary search over cell_offsets).
@@ -146,6 +148,36 @@ __device__ __forceinline__ int cell_key(const int (&n)[3], const double (&minc)[
     return key;
 }
 
+// Cell of a point under a SpatialHashingCellList: floor_to_int.(x ./ cell_size), no corner
+// and no +1 (nonperiodic_cell_coords, src/nhs_grid.jl:636-638).  ok is false when a cell
+// coordinate does not fit in Int32 (NaN and infinities included): the reference's
+// coordinates_flattened (src/cell_lists/spatial_hashing.jl:170-183) throws InexactError there.
+template <int D, class TC, class TCELL>
+__device__ __forceinline__ void hash_cell(const Geom& g, const TC* __restrict__ xp, int64_t (&c)[3], bool& ok) {
+    ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const TCELL q = (TCELL)xp[d] / (TCELL)g.cs;
+        const TCELL f = floor(q);
+        ok = ok && (f >= (TCELL)-2147483648.0) && (f <= (TCELL)2147483647.0);
+        c[d] = ok ? (int64_t)f : 0;
+    }
+}
+
+// spatial_hash (src/cell_lists/spatial_hashing.jl:150-164, Ihmsen et al. 2011) in Julia's
+// wrapping Int64 arithmetic, 0-based: mod(xor(i * 73856093, j * 19349663, k * 83492791), L).
+template <int D>
+__device__ __forceinline__ int spatial_hash(const int64_t (&c)[3], int list_size) {
+    uint64_t h = (uint64_t)c[0] * 73856093ull;
+    if (D >= 2) h ^= (uint64_t)c[1] * 19349663ull;
+    if (D >= 3) h ^= (uint64_t)c[2] * 83492791ull;
+    int64_t m = (int64_t)h % (int64_t)list_size;
+    if (m < 0) m += list_size;
+    return (int)m;
+}
+
 __device__ __forceinline__ int64_t sel_point(int64_t a, int64_t sel_first, const int32_t* sel_list) {
     return sel_list ? (int64_t)sel_list[a] : sel_first + a;
 }
@@ -165,7 +197,11 @@ __global__ __launch_bounds__(256) void k_bin(Geom g, const TC* __restrict__ y, i
         int64_t p = sel_point(a, sel_first, sel_list);
         bool ok;
         int k;
-        if (g.table) {
+        if (g.hash) {
+            int64_t c[3];
+            hash_cell<D, TC, TCELL>(g, y + (int64_t)D * p, c, ok);
+            k = spatial_hash<D>(c, g.list_size);
+        } else if (g.table) {
             const GeomEntry e = g.table[g.cloud_of_point[p]];
             k = e.cell_off + cell_key<D, TC, TCELL>(e.n, e.minc, g.csd, g, y + (int64_t)D * p, ok);
         } else {

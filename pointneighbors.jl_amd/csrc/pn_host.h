@@ -414,6 +414,14 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
         f.fsorted = g.fields.p;
         f.fstride = stride;
     }
+    if (g.geom.hash) {
+        using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
+        const TR rh = (TR)g.radius;
+        hipLaunchKernelGGL((pn::k_hash_traverse<D, TC, TR, TCELL, F>), dim3(blocks_for(Q->n_sel, 256)), dim3(256), 0,
+                           g.stream, g.geom, Q->view(), g.grid.view(), Q->n_sel, rh * rh, f);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    }
     // Upper bound on non-empty query cells; the kernel reads the exact count on device.
     int64_t nwork = std::min<int64_t>(g.geom.ncells, Q->n_sel);
     if (Q->stats_known) nwork = Q->stats.n_nonempty;

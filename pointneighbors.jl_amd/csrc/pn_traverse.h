@@ -330,6 +330,83 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
     }  // persistent loop over cells
 }
 
+// Traversal of a grid keyed by a SpatialHashingCellList (src/nhs_grid.jl:519-575 with the
+// collision checks of :479-510): one query per thread, queries in bucket order.  The 3^D
+// neighbouring cells are visited in CartesianIndices order (first dimension fastest); each
+// one's bucket is scanned and an accepted candidate counts only if its own cell is that
+// neighbouring cell.  That is the reference's check_collision; it applies the check only to
+// buckets that check_cell_collision flags, and for every other bucket it holds anyway.  So
+// each neighbour is visited exactly once, in the reference's cell order.
+template <int D, class TC, class TR, class TCELL, class F>
+__global__ __launch_bounds__(256) void k_hash_traverse(Geom g, SetView Q, SetView G, int64_t n_q, TR r2, F f) {
+    constexpr int NF = F::kFields;
+    constexpr int NS = D == 1 ? 3 : (D == 2 ? 9 : 27);
+    __shared__ int s_cnt[F::kFill ? 256 : 1];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_q) return;
+    const TC* __restrict__ gc = (const TC*)G.coords;
+    const TC* __restrict__ qcrd = (const TC*)Q.coords;
+    const TR* __restrict__ fs = (const TR*)f.fsorted;
+    TC xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = qcrd[(int64_t)d * Q.cap + k];
+    const int64_t orig_i = Q.sorted_idx[k];
+    int64_t c[3];
+    bool ok;
+    hash_cell<D, TC, TCELL>(g, xi, c, ok);  // binning rejected every point with !ok
+    const typename F::QState qst = f.query(orig_i);
+    typename F::Acc acc = f.init();
+    int* qcnt = nullptr;
+    if constexpr (F::kFill) {
+        s_cnt[threadIdx.x] = f.row_begin(orig_i);
+        f.row_end = f.row_begin(orig_i + 1);
+        qcnt = &s_cnt[threadIdx.x];
+    }
+    for (int o = 0; o < NS; ++o) {
+        int64_t nc[3] = {c[0], c[1], c[2]};
+        int t = o;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            nc[d] = c[d] + (t % 3) - 1;
+            t /= 3;
+        }
+        const int b = spatial_hash<D>(nc, g.list_size);
+        const int s1 = G.start[b + 1];
+        for (int src = G.start[b]; src < s1; ++src) {
+            TC xj[D];
+            TR pd[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                xj[d] = gc[(int64_t)d * G.cap + src];
+                pd[d] = (TR)(xi[d] - xj[d]);
+            }
+            TR d2 = pd[0] * pd[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                TR p2 = pd[d] * pd[d];
+                d2 = d2 + p2;
+            }
+            if (!(d2 <= r2)) continue;
+            int64_t cj[3];
+            bool okj;
+            hash_cell<D, TC, TCELL>(g, xj, cj, okj);
+            bool same = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) same = same && cj[d] == nc[d];
+            if (!same) continue;
+            int j = 0;
+            if constexpr (F::kIndex) j = G.sorted_idx[src];
+            TR fj[NF > 0 ? NF : 1];
+            if constexpr (NF > 0) {
+#pragma unroll
+                for (int fi = 0; fi < NF; ++fi) fj[fi] = fs[(int64_t)fi * f.fstride + src];
+            }
+            f.template pair<TR, D>(acc, qst, pd, d2, j, fj, qcnt);
+        }
+    }
+    if constexpr (!F::kFill) f.write(orig_i, qst, acc);
+}
+
 // Gathers per-neighbour fields into the grid's sorted order: out[f * stride + k] =
 // src_f[sorted_idx[k] * step_f + off_f] (a field may be one component of a D x N array).
 constexpr int MAX_FIELDS = 6;

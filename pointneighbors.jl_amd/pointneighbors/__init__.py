@@ -8,7 +8,7 @@ GPU (strides (1, D), the memory of Julia's column-major matrix).
 """
 
 from ._lib import ArgumentError, BoundsError, PointNeighborsError, load
-from .cell_lists import DynamicVectorOfVectors, FullGridCellList
+from .cell_lists import DynamicVectorOfVectors, FullGridCellList, SpatialHashingCellList
 from .neighborhood_search import (CountNeighbors, NeighborIndex, One, SumNeighborIndices,
                                   copy_neighborhood_search, foreach_neighbor,
                                   foreach_neighbor_unsafe, foreach_point_neighbor,
@@ -23,7 +23,7 @@ from .util import floor_to_int
 
 __all__ = [
     "ArgumentError", "BoundsError", "PointNeighborsError", "load",
-    "DynamicVectorOfVectors", "FullGridCellList",
+    "DynamicVectorOfVectors", "FullGridCellList", "SpatialHashingCellList",
     "CountNeighbors", "NeighborIndex", "One", "SumNeighborIndices",
     "copy_neighborhood_search", "foreach_neighbor", "foreach_neighbor_unsafe",
     "foreach_point_neighbor", "foreach_point_neighbor_unsafe", "freeze_neighborhood_search",

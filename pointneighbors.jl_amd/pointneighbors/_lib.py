@@ -57,6 +57,7 @@ SIGNATURES = {
     "pn_last_error": (ctypes.c_char_p, []),
     "pn_grid_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _p_dbl, _p_i64, _c_dbl, _c_i32,
                                 ctypes.POINTER(_vp)]),
+    "pn_grid_create_spatial_hash": (_c_int, [_c_int, _c_int, _c_int, _c_dbl, _c_i64, _c_i32, ctypes.POINTER(_vp)]),
     "pn_grid_destroy": (_c_int, [_vp]),
     "pn_grid_set_stream": (_c_int, [_vp, _vp]),
     "pn_grid_update": (_c_int, [_vp, _vp, _c_i64, _c_i64, _c_i64, _vp]),

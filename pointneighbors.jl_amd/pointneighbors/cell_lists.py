@@ -118,5 +118,57 @@ class FullGridCellList:
                 f"max_corner={self.max_corner.tolist()}, n_cells={self.n_cells})")
 
 
+def spatial_hash(cell, list_size) -> int:
+    """spatial_hash (src/cell_lists/spatial_hashing.jl:150-164, Ihmsen et al. 2011), 1-based:
+    mod(xor(i * 73856093, j * 19349663, k * 83492791), list_size) + 1 in wrapping Int64."""
+    m64 = (1 << 64) - 1
+    h = 0
+    for c, k in zip(cell, (73856093, 19349663, 83492791)):
+        h ^= (int(c) * k) & m64
+    if h >= 1 << 63:
+        h -= 1 << 64
+    return h % int(list_size) + 1
+
+
+class _HashMeta(type):
+    def __getitem__(cls, ndims):
+        return lambda **kw: cls(ndims, **kw)
+
+
+class SpatialHashingCellList(metaclass=_HashMeta):
+    """SpatialHashingCellList{NDIMS}(; list_size, backend, max_points_per_cell = 100) —
+    src/cell_lists/spatial_hashing.jl:1-183.  Cells of size search_radius cover all of R^D;
+    a cell's points live in bucket spatial_hash(cell) of `list_size` buckets (about twice the
+    number of points is the reference's recommendation), and `max_points_per_cell` bounds a
+    bucket.  Use as `SpatialHashingCellList[D](list_size=2 * n)`."""
+
+    def __init__(self, ndims, *, list_size, backend=DynamicVectorOfVectors, max_points_per_cell=100):
+        _check_backend(backend)
+        if int(ndims) not in (1, 2, 3):
+            raise ArgumentError("SpatialHashingCellList supports 1, 2 or 3 dimensions")
+        if int(list_size) < 1:
+            raise ArgumentError("list_size must be positive")
+        self.ndims = int(ndims)
+        self.list_size = int(list_size)
+        self.max_points_per_cell = int(max_points_per_cell)
+        self.is_template = False
+
+    def copy_cell_list(self, search_radius, periodic_box=None):
+        """src/cell_lists/spatial_hashing.jl:125-134: same list size and capacity."""
+        return SpatialHashingCellList(self.ndims, list_size=self.list_size,
+                                      max_points_per_cell=self.max_points_per_cell)
+
+    def bucket(self, cell) -> int:
+        """1-based bucket of a cell (the reference's `cell_list[cell]` reads this one)."""
+        return spatial_hash(cell, self.list_size)
+
+    def bounds_error_message(self, cell) -> str:
+        return ("InexactError: a cell coordinate does not fit in Int32 (coordinates_flattened, "
+                "src/cell_lists/spatial_hashing.jl:170-183); NaN or very large particle coordinates")
+
+    def __repr__(self):
+        return f"SpatialHashingCellList{{{self.ndims}}}(list_size={self.list_size})"
+
+
 def empty_template_error():
     raise PointNeighborsError("`search_radius` is not defined for this cell list")

@@ -14,7 +14,7 @@ import numpy as np
 from . import _device
 from ._lib import (PN_ERR_OUT_OF_BOUNDS, PN_QUERIES_BINNED, PN_QUERIES_GRID, ArgumentError,
                    BoundsError, PointNeighborsError, check, load)
-from .cell_lists import FullGridCellList
+from .cell_lists import FullGridCellList, SpatialHashingCellList
 from .util import FLOAT64_EPS, dtype_code, floor_to_int, promote, scalar_dtype, wrap_int64
 
 
@@ -112,8 +112,10 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             raise ArgumentError(
                 "the MI355X engine needs `cell_list = FullGridCellList(...)`; "
                 "DictionaryCellList is a CPU hash table")
-        if not isinstance(cell_list, FullGridCellList):
+        if not isinstance(cell_list, (FullGridCellList, SpatialHashingCellList)):
             raise ArgumentError(f"cell list {type(cell_list).__name__} is not supported")
+        if isinstance(cell_list, SpatialHashingCellList) and periodic_box is not None:
+            raise ArgumentError("a PeriodicBox with a SpatialHashingCellList is not supported by the MI355X engine")
         if cell_list.ndims != ndims:
             raise ArgumentError(f"a {ndims}D cell list is required for a GridNeighborhoodSearch{{{ndims}}}")
         if update_strategy is None:
@@ -176,8 +178,20 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             lib.pn_grid_destroy(self._handle)
             self._handle = None
         cl = self.cell_list
-        cell_dt = promote(coord_dtype, cl.corner_dtype)
         nd = self.ndims
+        if self._hashed():
+            h = ctypes.c_void_p()
+            check(lib.pn_grid_create_spatial_hash(nd, dtype_code(coord_dtype), dtype_code(self.radius_dtype),
+                                                  float(self.search_radius), cl.list_size,
+                                                  int(cl.max_points_per_cell), ctypes.byref(h)),
+                  "GridNeighborhoodSearch")
+            self._handle = h
+            self._coord_dtype = coord_dtype
+            self._cell_dtype = promote(coord_dtype, self.radius_dtype)
+            for k, v in self._tuning.items():
+                check(lib.pn_set_tuning(h, k.encode(), int(v)), "set_tuning")
+            return
+        cell_dt = promote(coord_dtype, cl.corner_dtype)
         mins = (ctypes.c_double * nd)(*[float(v) for v in cl.min_corner])
         ncs = (ctypes.c_int64 * nd)(*cl.n_cells)
         h = ctypes.c_void_p()
@@ -200,6 +214,9 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
     def _periodic(self):
         return self.periodic_box is not None and self.n_cells[0] > 0
 
+    def _hashed(self):
+        return isinstance(self.cell_list, SpatialHashingCellList)
+
     def set_tuning(self, key, value):
         """Autotuner knobs: "algorithm" (0: candidate-major cells, 1: per-query candidate
         slices), "persist_blocks" (traversal workgroups, multiple of 8), "chunk" (-1 per
@@ -216,6 +233,12 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         """cell_coords (src/nhs_grid.jl:622-628 with src/cell_lists/full_grid.jl:84-94) of
         one point given on the host, evaluated in the same types as the device."""
         cl = self.cell_list
+        if self._hashed():
+            # nonperiodic_cell_coords for a SpatialHashingCellList, src/nhs_grid.jl:636-638
+            ct = self._cell_dtype.type if self._handle is not None else promote(
+                np.asarray(coords).dtype, self.radius_dtype).type
+            with np.errstate(all="ignore"):
+                return tuple(floor_to_int(float(ct(coords[d]) / ct(self.search_radius))) for d in range(self.ndims))
         ct = self._cell_dtype.type if self._handle is not None else promote(
             np.asarray(coords).dtype, cl.corner_dtype).type
         out = []
@@ -339,7 +362,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         nb, ne, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(lib.pn_grid_stats(self._handle, ctypes.byref(nb), ctypes.byref(ne), ctypes.byref(mx)))
         dev = torch.device("cuda", torch.cuda.current_device())
-        C = self.cell_list.n_cells_total
+        C = self.cell_list.list_size if self._hashed() else self.cell_list.n_cells_total
         start = torch.empty(C + 1, dtype=torch.int32, device=dev)
         idx = torch.empty(max(nb.value, 1), dtype=torch.int32, device=dev)
         check(lib.pn_grid_copy_cells(self._handle, ctypes.c_void_p(start.data_ptr()),
@@ -350,8 +373,14 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
     def points_in_cell(self, cell):
         """Points of one cell (1-based cell tuple), src/nhs_grid.jl:593-597."""
         start, idx = self.cell_table()
-        c = self.cell_list.cell_index(cell) - 1
+        c = self._cell_row(cell)
         return idx[start[c]:start[c + 1]].tolist()
+
+    def _cell_row(self, cell):
+        """0-based row of the cell table holding `cell`: its linear index, or its bucket."""
+        if self._hashed():
+            return self.cell_list.bucket(cell) - 1
+        return self.cell_list.cell_index(cell) - 1
 
     def eachneighbor(self, coords):
         """Candidate points of the 3^D cells around `coords` (src/nhs_grid.jl:585-591)."""
@@ -361,7 +390,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         rng = [range(c - 1, c + 2) for c in cell]
         for nc in itertools.product(*reversed(rng)):
             nc = tuple(reversed(nc))
-            c = self.cell_list.cell_index(nc) - 1
+            c = self._cell_row(nc)
             out.extend(idx[start[c]:start[c + 1]].tolist())
         return out
 

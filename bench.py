@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr", "tlsph"], default="wcsph")
+    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr", "tlsph", "hash"], default="wcsph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -85,6 +85,9 @@ def main():
         return
     if args.workload == "tlsph":
         print(json.dumps(bench_tlsph(args, dev)))
+        return
+    if args.workload == "hash":
+        print(json.dumps(bench_hash(args, dev)))
         return
     pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
     n = pts.shape[0]
@@ -536,6 +539,46 @@ def bench_tlsph(args, dev):
         "finite": r32["finite"],
         "f64": dict(r64, speedup_vs_h100={"interaction": 6.187 / r64["interaction_ms"],  # plot_benchmarks.jl:75,83
                                           "deformation_grad": 1.423 / r64["deformation_grad_ms"]}),
+    }
+
+
+def bench_hash(args, dev):
+    """update! + neighbour count on the 1M-point Float32 WCSPH cloud (100^3 lattice, spacing
+    0.01, r = 0.024) with a SpatialHashingCellList of list_size 2N (the reference's
+    recommendation), beside the same steps with the FullGridCellList; counts must agree."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import synthetic
+    pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
+    n = pts.shape[0]
+    r = np.float32(2.0) * (np.float32(1.2) * np.float32(SPACING))
+    x = synthetic.to_device_coords(pts, dev)
+    lo, hi = pts.min(axis=0) - np.float32(3) * r, pts.max(axis=0) + np.float32(3) * r
+    grids = {
+        "spatial_hashing": pn.GridNeighborhoodSearch[3](
+            search_radius=r, n_points=n, cell_list=pn.SpatialHashingCellList[3](list_size=2 * n)),
+        "full_grid": pn.GridNeighborhoodSearch[3](
+            search_radius=r, n_points=n, cell_list=pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=r)),
+    }
+    k = max(2, min(args.steps, 20))
+    res, counts = {}, {}
+    for name, nhs in grids.items():
+        c = torch.zeros(n, dtype=torch.int32, device=dev)
+        f = pn.CountNeighbors(c, accumulate=False)
+        pn.initialize(nhs, x, x)
+        res[name + "_update_ms"] = _timed(lambda: pn.update(nhs, x, x), k)
+        res[name + "_count_ms"] = _timed(lambda: pn.foreach_point_neighbor(f, x, x, nhs), k)
+        counts[name] = c
+    pairs = int(counts["full_grid"].sum().item())
+    step = res["spatial_hashing_update_ms"] + res["spatial_hashing_count_ms"]
+    return {
+        "metric": "neighbour pairs/s, update! + count with a SpatialHashingCellList (1M 3D Float32)",
+        "value": pairs / (step * 1e-3), "unit": "pairs/s", "n_gpus": 1, "steps": k, "warmup": 2,
+        "ms_per_step": step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 jittered lattice)",
+        "config": {"workload": "hash-1m-3d-f32: 100^3 lattice, spacing 0.01, r = 0.024, list_size 2N", "n_points": n},
+        "pairs": pairs, "counts_equal_full_grid": bool(torch.equal(counts["spatial_hashing"], counts["full_grid"])),
+        **res,
     }
 
 

@@ -414,14 +414,6 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
         f.fsorted = g.fields.p;
         f.fstride = stride;
     }
-    if (g.geom.hash) {
-        using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
-        const TR rh = (TR)g.radius;
-        hipLaunchKernelGGL((pn::k_hash_traverse<D, TC, TR, TCELL, F>), dim3(blocks_for(Q->n_sel, 256)), dim3(256), 0,
-                           g.stream, g.geom, Q->view(), g.grid.view(), Q->n_sel, rh * rh, f);
-        PN_CHECK_LAUNCH();
-        return PN_OK;
-    }
     // Upper bound on non-empty query cells; the kernel reads the exact count on device.
     int64_t nwork = std::min<int64_t>(g.geom.ncells, Q->n_sel);
     if (Q->stats_known) nwork = Q->stats.n_nonempty;
@@ -429,6 +421,18 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
+    if (g.geom.hash) {  // a SpatialHashingCellList: the per-query kernel over buckets
+        using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
+        const bool large_h = g.chunk < 0 ? !F::kQueue : g.chunk == 1;
+        if (large_h)
+            hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::large, true, TCELL>), dim3(nblocks),
+                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        else
+            hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::small, true, TCELL>), dim3(nblocks),
+                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    }
     if (g.algorithm == 0 || g.geom.periodic) {  // only the cell kernel handles periodic boxes
         pn::PairCache pc{};
         int pcm = 0;  // pair cache mode of the launch

@@ -113,6 +113,81 @@ struct FillF : FunctorBase {
     __device__ void write(int64_t, const QState&, const Acc&) const {}
 };
 
+// One query of a grid keyed by a SpatialHashingCellList (src/nhs_grid.jl:519-575 with the
+// collision checks of :479-510), scanned by one thread.  The 3^D neighbouring cells are
+// visited in CartesianIndices order (first dimension fastest); each one's bucket is scanned
+// and an accepted candidate counts only if its own cell is that neighbouring cell.  That is
+// the reference's check_collision; it applies the check only to buckets that
+// check_cell_collision flags, and for every other bucket it holds anyway.  So each
+// neighbour is visited exactly once, in the reference's cell order.
+template <int D, class TC, class TR, class TCELL, class F>
+__device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, const SetView& G, TR r2, F& f,
+                                           int64_t k, int* cnt_slot) {
+    constexpr int NF = F::kFields;
+    constexpr int NS = D == 1 ? 3 : (D == 2 ? 9 : 27);
+    const TC* __restrict__ gc = (const TC*)G.coords;
+    const TC* __restrict__ qcrd = (const TC*)Q.coords;
+    const TR* __restrict__ fs = (const TR*)f.fsorted;
+    TC xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = qcrd[(int64_t)d * Q.cap + k];
+    const int64_t orig_i = Q.sorted_idx[k];
+    int64_t c[3];
+    bool ok;
+    hash_cell<D, TC, TCELL>(g, xi, c, ok);  // binning rejected every point with !ok
+    const typename F::QState qst = f.query(orig_i);
+    typename F::Acc acc = f.init();
+    int* qcnt = nullptr;
+    if constexpr (F::kFill) {
+        *cnt_slot = f.row_begin(orig_i);
+        f.row_end = f.row_begin(orig_i + 1);
+        qcnt = cnt_slot;
+    }
+    for (int o = 0; o < NS; ++o) {
+        int64_t nc[3] = {c[0], c[1], c[2]};
+        int t = o;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            nc[d] = c[d] + (t % 3) - 1;
+            t /= 3;
+        }
+        const int b = spatial_hash<D>(nc, g.list_size);
+        const int s1 = G.start[b + 1];
+        for (int src = G.start[b]; src < s1; ++src) {
+            TC xj[D];
+            TR pd[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                xj[d] = gc[(int64_t)d * G.cap + src];
+                pd[d] = (TR)(xi[d] - xj[d]);
+            }
+            TR d2 = pd[0] * pd[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                TR p2 = pd[d] * pd[d];
+                d2 = d2 + p2;
+            }
+            if (!(d2 <= r2)) continue;
+            int64_t cj[3];
+            bool okj;
+            hash_cell<D, TC, TCELL>(g, xj, cj, okj);
+            bool same = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) same = same && cj[d] == nc[d];
+            if (!same) continue;
+            int j = 0;
+            if constexpr (F::kIndex) j = G.sorted_idx[src];
+            TR fj[NF > 0 ? NF : 1];
+            if constexpr (NF > 0) {
+#pragma unroll
+                for (int fi = 0; fi < NF; ++fi) fj[fi] = fs[(int64_t)fi * f.fstride + src];
+            }
+            f.template pair<TR, D>(acc, qst, pd, d2, j, fj, qcnt);
+        }
+    }
+    if constexpr (!F::kFill) f.write(orig_i, qst, acc);
+}
+
 // One wavefront (= one workgroup) per non-empty query cell at a time; a persistent
 // grid of workgroups walks the non-empty cells (count read on the device, so launch
 // shapes do not depend on the data and can be captured in a graph).
@@ -123,9 +198,15 @@ struct FillF : FunctorBase {
 //     the chunk each; slice s covers [s*L, s*L + L) with L odd (conflict-free banks)
 //  4. slice partials are combined in LDS in slice order (deterministic) and written
 //     to the query's original index.
-template <int D, class TC, class TR, class F, int CH>
+//
+// HASH: the grid is keyed by a SpatialHashingCellList and a work item is a non-empty query
+// bucket.  When all its queries share one cell, the pieces are that cell's 3^D neighbouring
+// cells' buckets, and a staged candidate whose own cell is not its piece's cell becomes a
+// sentinel (hash_query's check, applied once per staged candidate).  A bucket whose queries
+// lie in several cells (a hash collision) is scanned query by query with hash_query.
+template <int D, class TC, class TR, class F, int CH, bool HASH = false, class TCELL = TR>
 __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G, TR r2, F f, int nblocks) {
-    constexpr int R = Rows<D>::R;
+    constexpr int R = HASH ? (D == 1 ? 3 : (D == 2 ? 9 : 27)) : Rows<D>::R;
     constexpr int NF = F::kFields;
     // A chunk holds at most cmax = CAP - 2S + 1 candidates, so the padded length
     // S * L with L = ceil(mc / S) | 1 <= mc + 2S - 1 never exceeds CAP.
@@ -150,6 +231,32 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
     for (int w = xcd * chunk + (int)(blockIdx.x >> 3); w < w_end; w += per_xcd) {
     const int qc = Q.nonempty[w];
     const int qs = Q.start[qc], qe = Q.start[qc + 1];
+    int64_t qcell[3] = {0, 0, 0};
+    bool uniform = true;
+    if constexpr (HASH) {
+        const TC* __restrict__ qx = (const TC*)Q.coords;
+        TC x0[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x0[d] = qx[(int64_t)d * Q.cap + qs];
+        bool ok;
+        hash_cell<D, TC, TCELL>(g, x0, qcell, ok);
+        for (int k = qs + lane; k < qe; k += WAVE) {
+            TC xk[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) xk[d] = qx[(int64_t)d * Q.cap + k];
+            int64_t ck[3];
+            hash_cell<D, TC, TCELL>(g, xk, ck, ok);
+#pragma unroll
+            for (int d = 0; d < D; ++d) uniform = uniform && ck[d] == qcell[d];
+        }
+        uniform = !__any(!uniform);
+    }
+    if (HASH && !uniform) {
+        if constexpr (HASH) {
+            for (int k = qs + lane; k < qe; k += WAVE)
+                hash_query<D, TC, TR, TCELL, F>(g, Q, G, r2, f, (int64_t)k, &s_cnt[F::kFill ? lane : 0]);
+        }
+    } else {
     int n0 = g.n[0], n1 = g.n[1];
     if (g.table) {
         const GeomEntry& e = g.table[cloud_of_cell(g, qc)];
@@ -160,12 +267,25 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
     // stencil rows
     int rb = 0, rlen = 0;
     if (lane < R) {
-        int off = 0;
-        if (D >= 2) off += ((lane % 3) - 1) * n0;
-        if (D >= 3) off += ((lane / 3) - 1) * n0 * n1;
-        const int base = qc - 1 + off;
-        rb = G.start[base];
-        rlen = G.start[base + 3] - rb;
+        if constexpr (HASH) {
+            int64_t nc[3] = {qcell[0], qcell[1], qcell[2]};
+            int t = lane;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                nc[d] = qcell[d] + (t % 3) - 1;
+                t /= 3;
+            }
+            const int b = spatial_hash<D>(nc, g.list_size);
+            rb = G.start[b];
+            rlen = G.start[b + 1] - rb;
+        } else {
+            int off = 0;
+            if (D >= 2) off += ((lane % 3) - 1) * n0;
+            if (D >= 3) off += ((lane / 3) - 1) * n0 * n1;
+            const int base = qc - 1 + off;
+            rb = G.start[base];
+            rlen = G.start[base + 3] - rb;
+        }
     }
     int rbeg[R], pre[R + 1];
     pre[0] = 0;
@@ -212,16 +332,34 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
                 Pack<TC, D> pk;
                 if (k < mc) {
                     const int gk = m0 + k;
-                    int rsel_beg = rbeg[0], rsel_pre = pre[0];
+                    int rsel_beg = rbeg[0], rsel_pre = pre[0], rsel_t = 0;
 #pragma unroll
                     for (int t = 1; t < R; ++t)
                         if (gk >= pre[t]) {
                             rsel_beg = rbeg[t];
                             rsel_pre = pre[t];
+                            rsel_t = t;
                         }
                     const int src = rsel_beg + (gk - rsel_pre);
 #pragma unroll
                     for (int d = 0; d < D; ++d) pk.v[d] = gc[(int64_t)d * G.cap + src];
+                    if constexpr (HASH) {
+                        // a candidate of another cell in this bucket never passes the test
+                        int64_t cj[3];
+                        bool okj;
+                        hash_cell<D, TC, TCELL>(g, pk.v, cj, okj);
+                        bool same = true;
+                        int t = rsel_t;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            same = same && cj[d] == qcell[d] + (t % 3) - 1;
+                            t /= 3;
+                        }
+                        if (!same) {
+#pragma unroll
+                            for (int d = 0; d < D; ++d) pk.v[d] = (TC)INFINITY;
+                        }
+                    }
                     if constexpr (F::kIndex) s_j[k] = G.sorted_idx[src];
                     if constexpr (NF > 0) {
 #pragma unroll
@@ -326,85 +464,9 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
             }
         }
     }
+    }  // uniform work item
     __syncthreads();
     }  // persistent loop over cells
-}
-
-// Traversal of a grid keyed by a SpatialHashingCellList (src/nhs_grid.jl:519-575 with the
-// collision checks of :479-510): one query per thread, queries in bucket order.  The 3^D
-// neighbouring cells are visited in CartesianIndices order (first dimension fastest); each
-// one's bucket is scanned and an accepted candidate counts only if its own cell is that
-// neighbouring cell.  That is the reference's check_collision; it applies the check only to
-// buckets that check_cell_collision flags, and for every other bucket it holds anyway.  So
-// each neighbour is visited exactly once, in the reference's cell order.
-template <int D, class TC, class TR, class TCELL, class F>
-__global__ __launch_bounds__(256) void k_hash_traverse(Geom g, SetView Q, SetView G, int64_t n_q, TR r2, F f) {
-    constexpr int NF = F::kFields;
-    constexpr int NS = D == 1 ? 3 : (D == 2 ? 9 : 27);
-    __shared__ int s_cnt[F::kFill ? 256 : 1];
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_q) return;
-    const TC* __restrict__ gc = (const TC*)G.coords;
-    const TC* __restrict__ qcrd = (const TC*)Q.coords;
-    const TR* __restrict__ fs = (const TR*)f.fsorted;
-    TC xi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) xi[d] = qcrd[(int64_t)d * Q.cap + k];
-    const int64_t orig_i = Q.sorted_idx[k];
-    int64_t c[3];
-    bool ok;
-    hash_cell<D, TC, TCELL>(g, xi, c, ok);  // binning rejected every point with !ok
-    const typename F::QState qst = f.query(orig_i);
-    typename F::Acc acc = f.init();
-    int* qcnt = nullptr;
-    if constexpr (F::kFill) {
-        s_cnt[threadIdx.x] = f.row_begin(orig_i);
-        f.row_end = f.row_begin(orig_i + 1);
-        qcnt = &s_cnt[threadIdx.x];
-    }
-    for (int o = 0; o < NS; ++o) {
-        int64_t nc[3] = {c[0], c[1], c[2]};
-        int t = o;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            nc[d] = c[d] + (t % 3) - 1;
-            t /= 3;
-        }
-        const int b = spatial_hash<D>(nc, g.list_size);
-        const int s1 = G.start[b + 1];
-        for (int src = G.start[b]; src < s1; ++src) {
-            TC xj[D];
-            TR pd[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                xj[d] = gc[(int64_t)d * G.cap + src];
-                pd[d] = (TR)(xi[d] - xj[d]);
-            }
-            TR d2 = pd[0] * pd[0];
-#pragma unroll
-            for (int d = 1; d < D; ++d) {
-                TR p2 = pd[d] * pd[d];
-                d2 = d2 + p2;
-            }
-            if (!(d2 <= r2)) continue;
-            int64_t cj[3];
-            bool okj;
-            hash_cell<D, TC, TCELL>(g, xj, cj, okj);
-            bool same = true;
-#pragma unroll
-            for (int d = 0; d < D; ++d) same = same && cj[d] == nc[d];
-            if (!same) continue;
-            int j = 0;
-            if constexpr (F::kIndex) j = G.sorted_idx[src];
-            TR fj[NF > 0 ? NF : 1];
-            if constexpr (NF > 0) {
-#pragma unroll
-                for (int fi = 0; fi < NF; ++fi) fj[fi] = fs[(int64_t)fi * f.fstride + src];
-            }
-            f.template pair<TR, D>(acc, qst, pd, d2, j, fj, qcnt);
-        }
-    }
-    if constexpr (!F::kFill) f.write(orig_i, qst, acc);
 }
 
 // Gathers per-neighbour fields into the grid's sorted order: out[f * stride + k] =

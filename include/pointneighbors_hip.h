@@ -89,7 +89,9 @@ int pn_grid_destroy(pn_grid_t grid);
  * PN_ERR_OUT_OF_BOUNDS, the reference's InexactError.  Every traversal entry point works on the
  * handle; a traversal scans the 3^D neighbouring cells' buckets and keeps a candidate only if
  * its own cell is the scanned cell, so neighbour sets equal those of a FullGridCellList.  The
- * cell type is the promotion of the coordinate and radius types.  No periodic box, no batching. */
+ * cell type is the promotion of the coordinate and radius types.  pn_grid_set_periodic applies:
+ * cells are then wrapped (periodic_cell_index) and distances take the minimum image.  No
+ * batching. */
 int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, double search_radius,
                                 int64_t list_size, int32_t max_points_per_cell, pn_grid_t* out);
 

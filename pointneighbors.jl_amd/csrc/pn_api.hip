@@ -96,14 +96,13 @@ int pn_grid_set_periodic(pn_grid_t grid, const double* cell_size, const int64_t*
                          const double* box_size) {
     if (!grid || !cell_size || !n_periodic || !box_size) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     pn_grid& g = *grid;
-    if (g.geom.hash) return fail(PN_ERR_UNSUPPORTED, "periodic boxes are not supported with a SpatialHashingCellList");
     if (g.geom.table) return fail(PN_ERR_UNSUPPORTED, "periodic boxes are not supported for batched clouds");
     for (int d = 0; d < g.nd; ++d) {
         if (n_periodic[d] < 3)
             return fail(PN_ERR_ARGUMENT,
                         "the `GridNeighborhoodSearch` needs at least 3 cells in each dimension when used with "
                         "periodicity. Please use no NHS for very small problems.");
-        if (n_periodic[d] > g.geom.n[d] - 2)
+        if (!g.geom.hash && n_periodic[d] > g.geom.n[d] - 2)
             return fail(PN_ERR_ARGUMENT, "the cell list has fewer than n_periodic + 2 cells in a dimension");
         if (!(cell_size[d] > 0.0) || !(box_size[d] > 0.0)) return fail(PN_ERR_ARGUMENT, "box and cell sizes must be positive");
     }

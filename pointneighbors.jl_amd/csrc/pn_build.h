@@ -148,22 +148,41 @@ __device__ __forceinline__ int cell_key(const int (&n)[3], const double (&minc)[
     return key;
 }
 
+// mod(c - 2, n) + 2 in wrapping Int64 arithmetic (periodic_cell_index, src/nhs_grid.jl:609-620).
+__device__ __forceinline__ int64_t hash_wrap(int64_t c, int n) {
+    int64_t m = (int64_t)((uint64_t)c - 2ull) % (int64_t)n;
+    if (m < 0) m += n;
+    return m + 2;
+}
+
 // Cell of a point under a SpatialHashingCellList: floor_to_int.(x ./ cell_size), no corner
 // and no +1 (nonperiodic_cell_coords, src/nhs_grid.jl:636-638).  ok is false when a cell
 // coordinate does not fit in Int32 (NaN and infinities included): the reference's
 // coordinates_flattened (src/cell_lists/spatial_hashing.jl:170-183) throws InexactError there.
-template <int D, class TC, class TCELL>
-__device__ __forceinline__ void hash_cell(const Geom& g, const TC* __restrict__ xp, int64_t (&c)[3], bool& ok) {
+// With a PeriodicBox the cell is wrapped, mod.(c .- 2, n_periodic) .+ 2 (periodic_cell_index,
+// src/nhs_grid.jl:609-620), which always fits: no error then, as in the reference.
+template <int D, class TC, class TCELL, bool PER>
+__device__ __forceinline__ void hash_cell_p(const Geom& g, const TC* __restrict__ xp, int64_t (&c)[3], bool& ok) {
     ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[d] = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const TCELL q = (TCELL)xp[d] / (TCELL)g.cs;
+        const TCELL q = (TCELL)xp[d] / (TCELL)g.csd[d];
         const TCELL f = floor(q);
-        ok = ok && (f >= (TCELL)-2147483648.0) && (f <= (TCELL)2147483647.0);
-        c[d] = ok ? (int64_t)f : 0;
+        if constexpr (PER) {
+            c[d] = hash_wrap(floor_to_int64(f), g.per[d]);
+        } else {
+            ok = ok && (f >= (TCELL)-2147483648.0) && (f <= (TCELL)2147483647.0);
+            c[d] = ok ? (int64_t)f : 0;
+        }
     }
+}
+
+template <int D, class TC, class TCELL>
+__device__ __forceinline__ void hash_cell(const Geom& g, const TC* __restrict__ xp, int64_t (&c)[3], bool& ok) {
+    if (g.periodic) hash_cell_p<D, TC, TCELL, true>(g, xp, c, ok);
+    else hash_cell_p<D, TC, TCELL, false>(g, xp, c, ok);
 }
 
 // spatial_hash (src/cell_lists/spatial_hashing.jl:150-164, Ihmsen et al. 2011) in Julia's

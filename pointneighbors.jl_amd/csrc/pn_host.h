@@ -424,12 +424,21 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     if (g.geom.hash) {  // a SpatialHashingCellList: the per-query kernel over buckets
         using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
         const bool large_h = g.chunk < 0 ? !F::kQueue : g.chunk == 1;
-        if (large_h)
-            hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::large, true, TCELL>), dim3(nblocks),
-                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
-        else
-            hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, Chunk<TC, F>::small, true, TCELL>), dim3(nblocks),
-                               dim3(pn::WAVE), 0, g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        auto launch_h = [&](auto chunk, auto perc) {
+            constexpr int CHK = decltype(chunk)::value;
+            constexpr bool PER = decltype(perc)::value;
+            hipLaunchKernelGGL((pn::k_traverse<D, TC, TR, F, CHK, true, TCELL, PER>), dim3(nblocks), dim3(pn::WAVE), 0,
+                               g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks);
+        };
+        using LargeC = pn::IntC<Chunk<TC, F>::large>;
+        using SmallC = pn::IntC<Chunk<TC, F>::small>;
+        if (g.geom.periodic) {
+            if (large_h) launch_h(LargeC{}, std::true_type{});
+            else launch_h(SmallC{}, std::true_type{});
+        } else {
+            if (large_h) launch_h(LargeC{}, std::false_type{});
+            else launch_h(SmallC{}, std::false_type{});
+        }
         PN_CHECK_LAUNCH();
         return PN_OK;
     }

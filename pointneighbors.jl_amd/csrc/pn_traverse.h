@@ -113,6 +113,33 @@ struct FillF : FunctorBase {
     __device__ void write(int64_t, const QState&, const Acc&) const {}
 };
 
+// compute_periodic_distance (src/neighborhood_search.jl:428-436) for a hash grid with a
+// PeriodicBox, in the operations of the cell kernel's periodic_fix: when d2 > r^2 and some
+// |pos_diff_d| > 0.49 L_d, pos_diff -= L .* rint.(pos_diff ./ L) and d2 is recomputed
+// (otherwise every rint is 0 and nothing changes).
+template <int D, class TR>
+__device__ __forceinline__ TR hash_min_image(const Geom& g, TR (&pd)[D], TR d2, TR r2) {
+    if (!(d2 > r2)) return d2;
+    bool need = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) need = need || (fabs(pd[d]) > (TR)0.49 * (TR)g.box[d]);
+    if (!need) return d2;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const TR box = (TR)g.box[d];
+        const TR k = rint(pd[d] / box);
+        const TR sft = box * k;
+        pd[d] = pd[d] - sft;
+    }
+    d2 = pd[0] * pd[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+        const TR p2 = pd[d] * pd[d];
+        d2 = d2 + p2;
+    }
+    return d2;
+}
+
 // One query of a grid keyed by a SpatialHashingCellList (src/nhs_grid.jl:519-575 with the
 // collision checks of :479-510), scanned by one thread.  The 3^D neighbouring cells are
 // visited in CartesianIndices order (first dimension fastest); each one's bucket is scanned
@@ -120,7 +147,7 @@ struct FillF : FunctorBase {
 // the reference's check_collision; it applies the check only to buckets that
 // check_cell_collision flags, and for every other bucket it holds anyway.  So each
 // neighbour is visited exactly once, in the reference's cell order.
-template <int D, class TC, class TR, class TCELL, class F>
+template <int D, class TC, class TR, class TCELL, class F, bool PER>
 __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, const SetView& G, TR r2, F& f,
                                            int64_t k, int* cnt_slot) {
     constexpr int NF = F::kFields;
@@ -134,7 +161,7 @@ __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, cons
     const int64_t orig_i = Q.sorted_idx[k];
     int64_t c[3];
     bool ok;
-    hash_cell<D, TC, TCELL>(g, xi, c, ok);  // binning rejected every point with !ok
+    hash_cell_p<D, TC, TCELL, PER>(g, xi, c, ok);  // binning rejected every point with !ok
     const typename F::QState qst = f.query(orig_i);
     typename F::Acc acc = f.init();
     int* qcnt = nullptr;
@@ -149,6 +176,7 @@ __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, cons
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             nc[d] = c[d] + (t % 3) - 1;
+            if constexpr (PER) nc[d] = hash_wrap(nc[d], g.per[d]);
             t /= 3;
         }
         const int b = spatial_hash<D>(nc, g.list_size);
@@ -167,10 +195,11 @@ __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, cons
                 TR p2 = pd[d] * pd[d];
                 d2 = d2 + p2;
             }
+            if constexpr (PER) d2 = hash_min_image<D, TR>(g, pd, d2, r2);
             if (!(d2 <= r2)) continue;
             int64_t cj[3];
             bool okj;
-            hash_cell<D, TC, TCELL>(g, xj, cj, okj);
+            hash_cell_p<D, TC, TCELL, PER>(g, xj, cj, okj);
             bool same = true;
 #pragma unroll
             for (int d = 0; d < D; ++d) same = same && cj[d] == nc[d];
@@ -204,7 +233,7 @@ __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, cons
 // cells' buckets, and a staged candidate whose own cell is not its piece's cell becomes a
 // sentinel (hash_query's check, applied once per staged candidate).  A bucket whose queries
 // lie in several cells (a hash collision) is scanned query by query with hash_query.
-template <int D, class TC, class TR, class F, int CH, bool HASH = false, class TCELL = TR>
+template <int D, class TC, class TR, class F, int CH, bool HASH = false, class TCELL = TR, bool HPER = false>
 __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G, TR r2, F f, int nblocks) {
     constexpr int R = HASH ? (D == 1 ? 3 : (D == 2 ? 9 : 27)) : Rows<D>::R;
     constexpr int NF = F::kFields;
@@ -239,13 +268,13 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
 #pragma unroll
         for (int d = 0; d < D; ++d) x0[d] = qx[(int64_t)d * Q.cap + qs];
         bool ok;
-        hash_cell<D, TC, TCELL>(g, x0, qcell, ok);
+        hash_cell_p<D, TC, TCELL, HPER>(g, x0, qcell, ok);
         for (int k = qs + lane; k < qe; k += WAVE) {
             TC xk[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) xk[d] = qx[(int64_t)d * Q.cap + k];
             int64_t ck[3];
-            hash_cell<D, TC, TCELL>(g, xk, ck, ok);
+            hash_cell_p<D, TC, TCELL, HPER>(g, xk, ck, ok);
 #pragma unroll
             for (int d = 0; d < D; ++d) uniform = uniform && ck[d] == qcell[d];
         }
@@ -254,7 +283,7 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
     if (HASH && !uniform) {
         if constexpr (HASH) {
             for (int k = qs + lane; k < qe; k += WAVE)
-                hash_query<D, TC, TR, TCELL, F>(g, Q, G, r2, f, (int64_t)k, &s_cnt[F::kFill ? lane : 0]);
+                hash_query<D, TC, TR, TCELL, F, HPER>(g, Q, G, r2, f, (int64_t)k, &s_cnt[F::kFill ? lane : 0]);
         }
     } else {
     int n0 = g.n[0], n1 = g.n[1];
@@ -273,6 +302,7 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 nc[d] = qcell[d] + (t % 3) - 1;
+                if constexpr (HPER) nc[d] = hash_wrap(nc[d], g.per[d]);
                 t /= 3;
             }
             const int b = spatial_hash<D>(nc, g.list_size);
@@ -347,12 +377,14 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
                         // a candidate of another cell in this bucket never passes the test
                         int64_t cj[3];
                         bool okj;
-                        hash_cell<D, TC, TCELL>(g, pk.v, cj, okj);
+                        hash_cell_p<D, TC, TCELL, HPER>(g, pk.v, cj, okj);
                         bool same = true;
                         int t = rsel_t;
 #pragma unroll
                         for (int d = 0; d < D; ++d) {
-                            same = same && cj[d] == qcell[d] + (t % 3) - 1;
+                            int64_t want = qcell[d] + (t % 3) - 1;
+                            if constexpr (HPER) want = hash_wrap(want, g.per[d]);
+                            same = same && cj[d] == want;
                             t /= 3;
                         }
                         if (!same) {
@@ -395,6 +427,7 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
                         TR p2 = pd[d] * pd[d];
                         d2 = d2 + p2;
                     }
+                    if constexpr (HASH && HPER) d2 = hash_min_image<D, TR>(g, pd, d2, r2);
                     return d2;
                 };
                 auto body = [&](int k, const TR (&pd)[D], TR d2) {

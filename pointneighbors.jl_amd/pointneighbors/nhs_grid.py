@@ -114,8 +114,6 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
                 "DictionaryCellList is a CPU hash table")
         if not isinstance(cell_list, (FullGridCellList, SpatialHashingCellList)):
             raise ArgumentError(f"cell list {type(cell_list).__name__} is not supported")
-        if isinstance(cell_list, SpatialHashingCellList) and periodic_box is not None:
-            raise ArgumentError("a PeriodicBox with a SpatialHashingCellList is not supported by the MI355X engine")
         if cell_list.ndims != ndims:
             raise ArgumentError(f"a {ndims}D cell list is required for a GridNeighborhoodSearch{{{ndims}}}")
         if update_strategy is None:
@@ -188,6 +186,11 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             self._handle = h
             self._coord_dtype = coord_dtype
             self._cell_dtype = promote(coord_dtype, self.radius_dtype)
+            if self._periodic():
+                cs = (ctypes.c_double * nd)(*[float(v) for v in self.cell_size])
+                npc = (ctypes.c_int64 * nd)(*self.n_cells)
+                bs = (ctypes.c_double * nd)(*[float(v) for v in self.periodic_box.size])
+                check(lib.pn_grid_set_periodic(h, cs, npc, bs), "GridNeighborhoodSearch")
             for k, v in self._tuning.items():
                 check(lib.pn_set_tuning(h, k.encode(), int(v)), "set_tuning")
             return
@@ -237,8 +240,14 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             # nonperiodic_cell_coords for a SpatialHashingCellList, src/nhs_grid.jl:636-638
             ct = self._cell_dtype.type if self._handle is not None else promote(
                 np.asarray(coords).dtype, self.radius_dtype).type
-            with np.errstate(all="ignore"):
-                return tuple(floor_to_int(float(ct(coords[d]) / ct(self.search_radius))) for d in range(self.ndims))
+            out = []
+            for d in range(self.ndims):
+                with np.errstate(all="ignore"):
+                    c = floor_to_int(float(ct(coords[d]) / ct(self.cell_size[d])))
+                if self._periodic():
+                    c = (wrap_int64(c - 2) % self.n_cells[d]) + 2
+                out.append(c)
+            return tuple(out)
         ct = self._cell_dtype.type if self._handle is not None else promote(
             np.asarray(coords).dtype, cl.corner_dtype).type
         out = []

@@ -13,8 +13,11 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def periodic_nhs(lo, hi, r, T=np.float64):
+def periodic_nhs(lo, hi, r, T=np.float64, hashed=False):
     pb = pn.PeriodicBox(min_corner=np.asarray(lo, dtype=T), max_corner=np.asarray(hi, dtype=T))
+    if hashed:  # a small table: wrapped cells collide in buckets
+        cl = pn.SpatialHashingCellList[len(lo)](list_size=97)
+        return pn.GridNeighborhoodSearch(len(lo), search_radius=T(r), periodic_box=pb, cell_list=cl)
     cl = pn.FullGridCellList(min_corner=pb.min_corner, max_corner=pb.max_corner, search_radius=T(r))
     return pn.GridNeighborhoodSearch(len(lo), search_radius=T(r), periodic_box=pb, cell_list=cl)
 
@@ -29,19 +32,21 @@ def oracle_for(orc, nhs, T):
     return g
 
 
-def test_reference_cases(orc, golden):
+@pytest.mark.parametrize("hashed", [False, True])
+def test_reference_cases(orc, golden, hashed):
     P = golden["periodic"]
     r = P["search_radius"]
     want = [[v - 1 for v in l] for l in P["neighbors"]]
     for case in P["cases"]:
         pts = np.asarray(case["coords"])
         x = dev(pts)
-        nhs = periodic_nhs(case["min_corner"], case["max_corner"], r)
+        nhs = periodic_nhs(case["min_corner"], case["max_corner"], r, hashed=hashed)
         pn.initialize(nhs, x, x)
         assert gpu_counts(nhs, x, x).tolist() == [len(w) for w in want], case["name"]
         assert gpu_sums(nhs, x, x).tolist() == [sum(w) for w in want], case["name"]
         p = pn.PrecomputedNeighborhoodSearch(pts.shape[1], search_radius=r, n_points=len(pts),
-                                             periodic_box=nhs.periodic_box)
+                                             periodic_box=nhs.periodic_box,
+                                             update_neighborhood_search=nhs if hashed else None)
         pn.initialize(p, x, x)
         assert [p.neighbor_lists[i] for i in range(len(pts))] == want, case["name"]
 
@@ -53,7 +58,8 @@ def test_reference_cases(orc, golden):
     (2, np.float32, ([-1.0, -1.0], [8.0, 16.0]), 2.9),   # 3 periodic cells in x
     (3, np.float32, ([0.0, 0.0, 0.0], [7.5, 7.5, 7.5]), 2.4),  # 3 cells per dimension
 ])
-def test_clouds_vs_oracle(orc, nd, T, box, r):
+@pytest.mark.parametrize("hashed", [False, True])
+def test_clouds_vs_oracle(orc, nd, T, box, r, hashed):
     rng = np.random.default_rng(101 + nd)
     lo = np.asarray(box[0], dtype=T)
     hi = np.asarray(box[1], dtype=T)
@@ -62,10 +68,10 @@ def test_clouds_vs_oracle(orc, nd, T, box, r):
     # mostly inside the box, some a little outside (cells wrap, coordinates do not)
     u = rng.uniform(-0.03, 1.03, size=(n, nd))
     pts = (lo + u * size).astype(T)
-    nhs = periodic_nhs(lo, hi, r, T)
+    nhs = periodic_nhs(lo, hi, r, T, hashed=hashed)
     x = dev(pts)
     pn.initialize(nhs, x, x)
-    og = oracle_for(orc, nhs, T)
+    og = oracle_for(orc, periodic_nhs(lo, hi, r, T), T)
     assert og.build(pts)[0] == orc.OK
     st, _, ocnt, osum, (orp, ocols) = og.traverse(pts, pts, lists=True)
     assert st == orc.OK

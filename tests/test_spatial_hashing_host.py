@@ -49,9 +49,11 @@ def test_cell_coordinates_and_api():
     assert nhs.host_cell_coords(np.array([-0.15, -0.05])) == (-2, -1)
     c2 = cl.copy_cell_list(0.5)
     assert (c2.list_size, c2.max_points_per_cell, c2.ndims) == (1, 100, 2)
-    with pytest.raises(pn.ArgumentError):
-        pn.GridNeighborhoodSearch(2, search_radius=1.0, cell_list=pn.SpatialHashingCellList[2](list_size=4),
-                                  periodic_box=pn.PeriodicBox(min_corner=[0.0, 0.0], max_corner=[4.0, 4.0]))
+    per = pn.GridNeighborhoodSearch(2, search_radius=1.0, cell_list=pn.SpatialHashingCellList[2](list_size=4),
+                                    periodic_box=pn.PeriodicBox(min_corner=[0.0, 0.0], max_corner=[4.0, 4.0]))
+    # periodic_cell_index of floor_to_int.(x ./ cell_size): 4 cells of size 1, wrapped to 2:5
+    assert per.host_cell_coords(np.array([0.5, 3.5])) == (4, 3)
+    assert per.host_cell_coords(np.array([-0.5, 4.5])) == (3, 4)
     with pytest.raises(pn.ArgumentError):
         pn.SpatialHashingCellList[2](list_size=0)
     cp = pn.copy_neighborhood_search(nhs, 0.3, 5)

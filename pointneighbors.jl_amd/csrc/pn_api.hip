@@ -488,6 +488,7 @@ int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, in
     if (!(penalty_alpha >= 0.0)) return fail(PN_ERR_ARGUMENT, "penalty_alpha must be non-negative");
     const bool pen = penalty_alpha > 0.0;
     if (pen && (!x || !rho || !F)) return fail(PN_ERR_ARGUMENT, "the penalty force needs x, rho and F");
+    if (workspace && ((uintptr_t)workspace & 15u) != 0) return fail(PN_ERR_ARGUMENT, "workspace must be 16-byte aligned");
     if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
         return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
     if (sel_count <= 0) return PN_OK;

@@ -219,5 +219,17 @@ def test_interaction_argument_errors():
         sph.tlsph_interaction(p, Xd, Xd, cuda(m), cuda(rho), Pr[:, :2], h)
     with pytest.raises(pn.ArgumentError):
         sph.tlsph_interaction(p, Xd, Xd, cuda(m), cuda(rho), Pr.double(), h)
+    # a workspace off 16-byte alignment is refused before any launch (C-ABI status)
+    import ctypes
+    from pointneighbors._lib import PN_ERR_ARGUMENT, load
+    nl = p.neighbor_lists
+    ws = torch.empty(n * 32 + 8, dtype=torch.int32, device="cuda")
+    md, rd, dv = cuda(m), cuda(rho), torch.zeros((n, nd), dtype=torch.float32, device="cuda")
+    Pc = Pr.contiguous()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = load().pn_list_tlsph_interaction(vp(nl.row_ptr), vp(nl.col_idx), nl.n, 0, nd, 0, 1, float(h), vp(Xd), vp(Xd),
+                                          vp(md), vp(rd), vp(Pc), vp(Pc), 1.0, 0.1, 0, n, None, vp(dv), 0,
+                                          ctypes.c_void_p(ws.data_ptr() + 4), None)
+    assert st == PN_ERR_ARGUMENT
     with pytest.raises(pn.ArgumentError):  # lists of another cloud
         sph.tlsph_interaction(p, Xd[:, :-1], Xd[:, :-1], cuda(m[:-1]), cuda(rho[:-1]), Pr[:-1], h)

@@ -299,31 +299,65 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32
     }
     unsigned long long total;
     block_excl_scan(s, sh, total);
-    if (threadIdx.x == 0) {
-        block_sums[blockIdx.x] = total;
-        atomicAdd(&st->total64, total & 0xffffffffull);
-    }
-    // wave max of the populations, one atomic per wavefront
+    // block max of the populations; block sums and maxima go to k_scan_top (no same-address
+    // atomics: with millions of cells or buckets they serialised the pass)
+    __shared__ int shm[SCAN_THREADS / WAVE];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0 && mx > 0) atomicMax(&st->max_count, mx);
+    if ((threadIdx.x & (WAVE - 1)) == 0) shm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = shm[0];
+#pragma unroll
+        for (int i = 1; i < SCAN_THREADS / WAVE; ++i) m = max(m, shm[i]);
+        block_sums[blockIdx.x] = total;
+        block_sums[gridDim.x + blockIdx.x] = (unsigned long long)(uint32_t)m;
+    }
 }
 
-// Single workgroup: exclusive scan of the block sums, writes the totals.
+// Single workgroup: exclusive scan of the block sums (block_sums[0, nblocks)), writes the
+// totals, the 64-bit sum of all counts and the largest count (block maxima at
+// block_sums[nblocks, 2 nblocks)).
 static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long long* __restrict__ block_sums,
                                                            int64_t nblocks, int32_t* __restrict__ start,
                                                            int64_t C, DevStatus* st) {
     __shared__ unsigned long long sh[SCAN_THREADS / WAVE];
-    unsigned long long carry = 0;
+    __shared__ unsigned long long sh_sum[SCAN_THREADS / WAVE];
+    __shared__ int sh_max[SCAN_THREADS / WAVE];
+    unsigned long long carry = 0, sum64 = 0;
+    int mx = 0;
     for (int64_t b0 = 0; b0 < nblocks; b0 += SCAN_THREADS) {
         int64_t b = b0 + threadIdx.x;
         unsigned long long v = b < nblocks ? block_sums[b] : 0ull;
+        if (b < nblocks) {
+            sum64 += v & 0xffffffffull;
+            mx = max(mx, (int)(uint32_t)block_sums[nblocks + b]);
+        }
         unsigned long long total;
         unsigned long long ex = block_excl_scan(v, sh, total);
         if (b < nblocks) block_sums[b] = carry + ex;
         carry += total;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sum64 += __shfl_xor(sum64, o, WAVE);
+        mx = max(mx, __shfl_xor(mx, o, WAVE));
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        sh_sum[threadIdx.x >> 6] = sum64;
+        sh_max[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        int m = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_THREADS / WAVE; ++i) {
+            tot += sh_sum[i];
+            m = max(m, sh_max[i]);
+        }
+        st->total64 = tot;
+        st->max_count = m;
         start[C] = (int32_t)(uint32_t)(carry & 0xffffffffull);
         st->n_binned = (int)(uint32_t)(carry & 0xffffffffull);
         st->n_nonempty = (int)(uint32_t)(carry >> 32);

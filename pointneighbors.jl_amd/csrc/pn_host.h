@@ -78,7 +78,7 @@ struct Scanner {
     int run(const int32_t* counts, int64_t n, int32_t* start, int32_t* nonempty, pn::DevStatus* st,
             hipStream_t stream) {
         const int64_t nb = std::max<int64_t>(1, (n + pn::SCAN_TILE - 1) / pn::SCAN_TILE);
-        int rc = block_sums.ensure((size_t)nb * 8, stream);
+        int rc = block_sums.ensure((size_t)nb * 16, stream);  // block sums, then block maxima
         if (rc) return rc;
         auto* bs = block_sums.as<unsigned long long>();
         hipLaunchKernelGGL(pn::k_scan_reduce, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts,

@@ -92,6 +92,8 @@ int pn_grid_destroy(pn_grid_t grid);
  * cell type is the promotion of the coordinate and radius types.  pn_grid_set_periodic applies:
  * cells are then wrapped (periodic_cell_index) and distances take the minimum image.  No
  * batching. */
+/* (Both grid constructors accept max_points_per_cell up to INT32_MAX / 27 = 79536431: a 3^D
+ * stencil's candidates are counted in int32.) */
 int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, double search_radius,
                                 int64_t list_size, int32_t max_points_per_cell, pn_grid_t* out);
 

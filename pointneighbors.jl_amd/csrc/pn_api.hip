@@ -26,6 +26,8 @@ int pn_grid_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
     if (!(search_radius > 0.0) || !std::isfinite(search_radius))
         return fail(PN_ERR_ARGUMENT, "search_radius must be positive and finite");
     if (max_points_per_cell < 1) return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be positive");
+    if (max_points_per_cell > INT32_MAX / 27)  // a 3^D stencil's candidates are counted in int32
+        return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be at most 79536431");
     int64_t C = 1;
     for (int d = 0; d < ndims; ++d) {
         if (n_cells[d] < 3) return fail(PN_ERR_ARGUMENT, "every dimension needs at least 3 cells");
@@ -67,6 +69,8 @@ int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, do
     if (!(search_radius > 0.0) || !std::isfinite(search_radius))
         return fail(PN_ERR_ARGUMENT, "search_radius must be positive and finite");
     if (max_points_per_cell < 1) return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be positive");
+    if (max_points_per_cell > INT32_MAX / 27)  // a 3^D stencil's candidates are counted in int32
+        return fail(PN_ERR_ARGUMENT, "max_points_per_cell must be at most 79536431");
     if (list_size < 1 || list_size > INT32_MAX - 1) return fail(PN_ERR_ARGUMENT, "list_size must be in 1:2^31-2");
     pn_grid* g = new pn_grid();
     g->nd = ndims;

@@ -169,6 +169,16 @@ def test_library_argument_errors_without_gpu():
     assert lib.pn_count_neighbors(h, 0, None, 0) == 1
     assert lib.pn_count_neighbors(h, 0, ctypes.c_void_p(8), 0) == 5  # not initialized
     assert lib.pn_grid_destroy(h) == 0
+    # the stencil's candidate count must fit in int32
+    st = lib.pn_grid_create(2, 0, 0, 0, mins, ncs, 0.1, 2**31 // 27 + 1, ctypes.byref(h))
+    assert st == 1 and "max_points_per_cell" in lib.pn_last_error().decode()
+    # SpatialHashingCellList handles: argument checks before any device work
+    st = lib.pn_grid_create_spatial_hash(3, 0, 0, 0.1, 0, 100, ctypes.byref(h))
+    assert st == 1 and "list_size" in lib.pn_last_error().decode()
+    st = lib.pn_grid_create_spatial_hash(3, 0, 0, -1.0, 10, 100, ctypes.byref(h))
+    assert st == 1 and "search_radius" in lib.pn_last_error().decode()
+    st = lib.pn_grid_create_spatial_hash(3, 0, 0, 0.1, 10, 2**31 // 27 + 1, ctypes.byref(h))
+    assert st == 1 and "max_points_per_cell" in lib.pn_last_error().decode()
 
 
 def test_traversal_lds_padding_fits():

@@ -124,3 +124,32 @@ def test_sph_density_matches_full_grid(dt, tol):
         pn.foreach_point_neighbor(sph.SPHDensity(rho, mass, float(h), "cubic_spline"), x, x, nhs)
         out.append(rho)
     assert torch.allclose(out[0], out[1], rtol=tol, atol=0)
+
+
+def test_stale_coordinates_follow_current_cells():
+    """Points moved in place without update!: buckets come from the last update!, and a
+    candidate counts only if its current cell is the scanned cell (the reference's
+    check_collision on every bucket, as its parallel build flags them all).  Expected
+    counts from that loop restated in numpy (src/nhs_grid.jl:519-575)."""
+    r = 2.5
+    a = cloud((25, 25), 1)
+    b = a + (synthetic.uniform(9, a.size).reshape(a.shape) - 0.5) * 0.6
+    n = len(a)
+    L = 2 * n
+    nhs = hash_nhs(2, r, n, L)
+    x = dev(a)
+    pn.initialize(nhs, x, x)
+    x.copy_(dev(b))
+    cell = lambda p: tuple(int(v) for v in np.floor(p / r))
+    old_bucket = np.array([spatial_hash(cell(p), L) for p in a])
+    cur_cell = [cell(p) for p in b]
+    want = np.zeros(n, dtype=np.int64)
+    for q in range(n):
+        cq = cell(b[q])
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                nc = (cq[0] + dx, cq[1] + dy)
+                for p in np.nonzero(old_bucket == spatial_hash(nc, L))[0]:
+                    if cur_cell[p] == nc and np.sum((b[q] - b[p]) ** 2) <= r * r:
+                        want[q] += 1
+    assert np.array_equal(gpu_counts(nhs, x, x), want)

@@ -721,8 +721,9 @@ template <int D, class T>
 __global__ __launch_bounds__(256) void k_tlsph_pk1_rho2(const T* __restrict__ F, const T* __restrict__ L,
                                                         const T* __restrict__ rho, int64_t n, T lambda, T mu,
                                                         T* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // every thread reaches the barrier below; past the end it recomputes point n - 1 (n >= 1),
+    // whose copy lies outside the block's valid range and is not stored
+    const int64_t i = min((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n - 1);
     T f[D * D], E[D * D], S[D * D], P[D * D];
 #pragma unroll
     for (int k = 0; k < D * D; ++k) f[k] = F[(int64_t)D * D * i + k];
@@ -753,6 +754,10 @@ __global__ __launch_bounds__(256) void k_tlsph_pk1_rho2(const T* __restrict__ F,
         }
     const T ri = rho[i];
     const T s = (T)1 / (ri * ri);
+    // the block's D*D*256 outputs are staged in LDS and stored contiguously: per-thread stores
+    // D*D values apart left partial cache lines and wrote 2.7-4x the output bytes to HBM
+    __shared__ T s_out[256 * D * D];
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
@@ -765,8 +770,11 @@ __global__ __launch_bounds__(256) void k_tlsph_pk1_rho2(const T* __restrict__ F,
             } else {
                 v = P[r + D * c];
             }
-            out[(int64_t)D * D * i + r + D * c] = v * s;
+            s_out[D * D * threadIdx.x + r + D * c] = v * s;
         }
+    __syncthreads();
+    const int64_t m = min((int64_t)blockDim.x, n - i0) * D * D;  // valid outputs of this block
+    for (int64_t t = threadIdx.x; t < m; t += blockDim.x) out[D * D * i0 + t] = s_out[t];
 }
 
 }  // namespace pn

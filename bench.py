@@ -49,7 +49,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr", "tlsph", "hash"], default="wcsph")
+    ap.add_argument("--workload", choices=["wcsph", "count", "batch", "large", "csr", "tlsph", "hash",
+                                           "trixi-wcsph", "nbody", "update"], default="wcsph")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -88,6 +89,15 @@ def main():
         return
     if args.workload == "hash":
         print(json.dumps(bench_hash(args, dev)))
+        return
+    if args.workload == "trixi-wcsph":
+        print(json.dumps(bench_trixi_wcsph(args, dev)))
+        return
+    if args.workload == "nbody":
+        print(json.dumps(bench_nbody(args, dev)))
+        return
+    if args.workload == "update":
+        print(json.dumps(bench_update(args, dev)))
         return
     pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
     n = pts.shape[0]
@@ -471,6 +481,160 @@ def bench_csr(args, dev):
     }
 
 
+def reference_cloud(side, search_radius_factor, seed=1):
+    """The reference's benchmark cloud (test/point_cloud.jl:4-57 as driven by
+    benchmarks/run_benchmarks.jl:60-113): a side^3 unit lattice (1-based), jittered by
+    N(0, 0.05^2), stably sorted by the cells (size search_radius_factor) of that first jitter
+    with the first dimension slowest, jittered again by N(0, 0.05^2); float64, lattice units.
+    Julia's RNG stream cannot be reproduced; the jitter comes from numpy's seeded generator."""
+    from pointneighbors import synthetic
+    rng = np.random.default_rng(seed)
+    X = synthetic.jittered_lattice((side,) * 3, 1.0, 0.0, dtype=np.float64) + 1.0
+    X = X + 0.05 * rng.standard_normal(X.shape)
+    cells = np.floor(X / float(search_radius_factor)).astype(np.int64)
+    X = X + 0.05 * rng.standard_normal(X.shape)
+    return X[np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))]
+
+
+H100_WCSPH_MS = {"f32": 3.407714, "f64": 6.047292}  # benchmarks/plot_benchmarks.jl:67-68 (1,030,301 points)
+
+
+def bench_trixi_wcsph(args, dev):
+    """TrixiParticles' WCSPH interact! on the cloud of the reference's published WCSPH runs
+    (benchmark_wcsph, benchmarks/smoothed_particle_hydrodynamics.jl:44-102, run by
+    run_benchmark_full_grid(benchmark_wcsph, (10, 10, 10), 9, search_radius_factor = 3.0f0),
+    benchmarks/plot_benchmarks.jl:53-56): 101^3 = 1,030,301 points rescaled by the domain size
+    102, r = 3/102, FullGridCellList over [0, 1]^3, WendlandC2 with h = r/2, density
+    1000 + U[0, 1), mass 0.1 * spacing, velocity 0, StateEquationCole(c = 10, exponent 1),
+    ArtificialViscosityMonaghan(alpha = 0.02), DensityDiffusionMolteniColagrossi(delta = 0.1).
+    Times one interact! per call (dv accumulates, the pressure precomputed), as the reference
+    does, in Float32 (the headline) and Float64."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import synthetic
+    side = 101
+    base = reference_cloud(side, 3.0)
+    n = len(base)
+    k = max(5, min(args.steps, 50))
+    gen = np.random.default_rng(2)
+    rand = gen.random(n)
+
+    def run(dt, tdt):
+        domain = dt(side + 1)
+        X = (base.astype(dt) / domain).astype(dt)
+        r = dt(3.0) / domain
+        spacing = r / dt(3)
+        h = float(r) / 2
+        cl = pn.FullGridCellList(min_corner=np.zeros(3, dtype=dt), max_corner=np.ones(3, dtype=dt), search_radius=r)
+        nhs = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
+        x = synthetic.to_device_coords(X, dev)
+        pn.initialize(nhs, x, x)
+        rho = torch.as_tensor((dt(1000) + rand.astype(dt)).astype(dt), device=dev)
+        mass = torch.full((n,), float(dt(0.1) * spacing), dtype=tdt, device=dev)
+        v = torch.zeros((n, 4), dtype=tdt, device=dev)
+        v[:, 3] = rho
+        v = v.t()
+        p = pn.state_equation_cole(rho, 1000.0, 10.0, 1.0)
+        dv = torch.zeros_like(v)
+        f = pn.WCSPHInteraction(dv, v, p, mass, h, 10.0)
+        ms = _timed(lambda: pn.foreach_point_neighbor(f, x, x, nhs), k)
+        c = torch.zeros(n, dtype=torch.int32, device=dev)
+        pn.foreach_point_neighbor(pn.CountNeighbors(c), x, x, nhs)
+        pairs = int(c.sum().item())
+        return {"interact_ms": ms, "pairs": pairs, "pairs_per_s": pairs / (ms * 1e-3),
+                "finite": bool(torch.isfinite(dv).all().item()), "speedup_vs_h100": H100_WCSPH_MS[str(dt.__name__).replace("float", "f")] / ms}
+
+    r32 = run(np.float32, torch.float32)
+    r64 = run(np.float64, torch.float64)
+    return {
+        "metric": "WCSPH interact! time (benchmark_wcsph, 1,030,301 points)", "value": r32["interact_ms"],
+        "unit": "ms", "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": r32["interact_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": r32["interact_ms"] / H100_WCSPH_MS["f32"],
+        "dtype": "f32", "data": "synthetic (the reference's jittered, cell-sorted lattice; numpy RNG)",
+        "config": {"workload": "trixi-wcsph-101cube-3d: 101^3 lattice / 102, r = 3/102, h = r/2, WendlandC2, "
+                               "ContinuityDensity, Cole EOS (c 10, exponent 1), Monaghan alpha 0.02, "
+                               "Molteni-Colagrossi delta 0.1", "n_points": n},
+        "f32": r32, "f64": r64,
+        "h100_ms": H100_WCSPH_MS,
+    }
+
+
+def bench_nbody(args, dev):
+    """benchmark_n_body (benchmarks/n_body.jl:34-51) on the reference's published cloud
+    (101^3 / 102, r = 3/102, Float32, FullGridCellList over [0, 1]^3): dv .= 0, then the n-body
+    closure over all neighbours; mass 1e10 (1 + U[0, 1)), G = 6.6743e-11."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import synthetic
+    side = 101
+    X = (reference_cloud(side, 3.0).astype(np.float32) / np.float32(side + 1)).astype(np.float32)
+    n = len(X)
+    r = np.float32(3.0) / np.float32(side + 1)
+    cl = pn.FullGridCellList(min_corner=np.zeros(3, np.float32), max_corner=np.ones(3, np.float32), search_radius=r)
+    nhs = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
+    x = synthetic.to_device_coords(X, dev)
+    pn.initialize(nhs, x, x)
+    mass = torch.as_tensor((1e10 * (np.random.default_rng(3).random(n) + 1)).astype(np.float32), device=dev)
+    dv = torch.zeros((n, 3), dtype=torch.float32, device=dev).t()
+    f = pn.NBodyAcceleration(dv, mass, 6.6743e-11, accumulate=False)  # dv .= 0 then +=
+    k = max(5, min(args.steps, 50))
+    ms = _timed(lambda: pn.foreach_point_neighbor(f, x, x, nhs), k)
+    c = torch.zeros(n, dtype=torch.int32, device=dev)
+    pn.foreach_point_neighbor(pn.CountNeighbors(c), x, x, nhs)
+    pairs = int(c.sum().item())
+    return {
+        "metric": "n-body closure time (benchmark_n_body, 1,030,301 points)", "value": ms, "unit": "ms",
+        "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": ms, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (the reference's jittered, cell-sorted lattice)",
+        "config": {"workload": "nbody-101cube-3d-f32: 101^3 lattice / 102, r = 3/102", "n_points": n},
+        "pairs": pairs, "pairs_per_s": pairs / (ms * 1e-3), "finite": bool(torch.isfinite(dv).all().item()),
+    }
+
+
+def bench_update(args, dev):
+    """benchmark_update_alternating (benchmarks/update.jl:28-49) on the 1M-point WCSPH cloud:
+    coordinates2 = coordinates perturbed by N(0, (4e-4 r)^2); each call does update! with
+    coordinates then update! with coordinates2 (each a full counting-sort rebuild, checked for
+    the reference's errors); reported per update!.  initialize! (benchmark_initialize) beside it."""
+    import torch
+    import pointneighbors as pn
+    from pointneighbors import synthetic
+    pts = synthetic.jittered_lattice(SHAPE, SPACING, JITTER, dtype=np.float32, seed=1)
+    n = pts.shape[0]
+    h = np.float32(1.2) * np.float32(SPACING)
+    r = np.float32(2.0) * h
+    lo = pts.min(axis=0) - np.float32(3) * r
+    hi = pts.max(axis=0) + np.float32(3) * r
+    cl = pn.FullGridCellList(min_corner=lo, max_corner=hi, search_radius=r)
+    nhs = pn.GridNeighborhoodSearch[3](search_radius=r, n_points=n, cell_list=cl)
+    x = synthetic.to_device_coords(pts, dev)
+    pts2 = (pts + np.random.default_rng(4).standard_normal(pts.shape) * (4e-4 * float(r))).astype(np.float32)
+    x2 = synthetic.to_device_coords(pts2, dev)
+    pn.initialize(nhs, x, x)
+    k = max(5, min(args.steps, 50))
+
+    def alternating():
+        pn.update(nhs, x, x)
+        pn.update(nhs, x, x2)
+
+    pair_ms = _timed(alternating, k)
+    init_ms = _timed(lambda: pn.initialize(nhs, x, x), k)
+    s1, i1 = nhs.cell_table()
+    pn.update(nhs, x, x2)
+    s2, _ = nhs.cell_table()
+    moved = int(np.abs(np.diff(s1) - np.diff(s2)).sum() // 2)
+    return {
+        "metric": "update! time (benchmark_update_alternating, 1M 3D Float32)", "value": 0.5 * pair_ms,
+        "unit": "ms", "n_gpus": 1, "steps": k, "warmup": 2, "ms_per_step": 0.5 * pair_ms,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice, N(0, (4e-4 r)^2) perturbation)",
+        "config": {"workload": "update-1m-3d-f32: 100^3 lattice, spacing 0.01, r = 0.024, alternating clouds",
+                   "n_points": n},
+        "update_ms": 0.5 * pair_ms, "initialize_ms": init_ms, "points_per_s": n / (0.5 * pair_ms * 1e-3),
+        "cell_changes_lower_bound": moved,
+    }
+
+
 def bench_tlsph(args, dev):
     """TLSPH on the cloud of the reference's published TLSPH runs: 101^3 = 1,030,301 points, a
     unit lattice jittered twice by N(0, 0.05^2) and stably sorted by the cells (size 3) of the
@@ -487,12 +651,7 @@ def bench_tlsph(args, dev):
     from pointneighbors import sph, synthetic
     side = 101
     r = np.float32(3.0)
-    rng = np.random.default_rng(1)
-    X = synthetic.jittered_lattice((side,) * 3, 1.0, 0.0, dtype=np.float64)
-    X = X + 0.05 * rng.standard_normal(X.shape)
-    cells = np.floor(X / float(r)).astype(np.int64)  # sorted by these cells, first dimension slowest
-    X = X + 0.05 * rng.standard_normal(X.shape)
-    X = X[np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))].astype(np.float32)
+    X = reference_cloud(side, 3.0).astype(np.float32)
     n = len(X)
     h = float(r) / 2
     rho0, m0, E, nu, alpha = 1000.0, 0.1, 1.4e6, 0.4, 0.1

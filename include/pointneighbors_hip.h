@@ -168,6 +168,26 @@ int pn_count_neighbors(pn_grid_t grid, int queries, int32_t* counts, int accumul
 /* mapreduce_neighbor(+, ...) of (j + index_base) (test/neighborhood_search.jl:389-399). */
 int pn_sum_neighbor_indices(pn_grid_t grid, int queries, int64_t* sums, int index_base,
                             int accumulate);
+/* mapreduce_neighbor(f, op, ...; init) of every query point (src/neighborhood_search.jl:340-383)
+ * in double: value[i] = op over the neighbours of f(i, j, pos_diff, d), count[i] = number of
+ * neighbours (an empty neighbourhood leaves value[i] at op's identity: the caller returns
+ * init).  fn: 0 one, 1 neighbour index j + index_base, 2 distance, 3 squared distance,
+ * 4 + k component k of pos_diff; op: 0 +, 1 min, 2 max.  Overwrites value and count. */
+int pn_mapreduce_neighbors(pn_grid_t grid, int queries, int fn, int op, int index_base, double* value,
+                           int32_t* count);
+/* The n-body closure of benchmarks/n_body.jl:34-51: for every neighbour j at distance
+ * d >= sqrt(eps), dv[:, i] += -G mass[j] pos_diff / d^3.  dv: D x N (coordinates' layout);
+ * mass per grid point.  Coordinates and radius of one type, 2D or 3D. */
+int pn_nbody_accel(pn_grid_t grid, int queries, double G, const void* mass, void* dv, int accumulate);
+/* TrixiParticles' fluid-fluid interact! as timed by benchmark_wcsph
+ * (benchmarks/smoothed_particle_hydrodynamics.jl:44-102): WCSPH with ContinuityDensity, the
+ * pressure `pressure` precomputed, ArtificialViscosityMonaghan(alpha, beta, epsilon) and
+ * DensityDiffusionMolteniColagrossi(delta), smoothing length h, sound speed c.  v and dv are
+ * (D+1) x N matrices: velocity, then density.  dv accumulates when accumulate != 0 (interact!
+ * does).  Queries are the grid's own points (PN_QUERIES_GRID). */
+int pn_wcsph_interact(pn_grid_t grid, int kernel, double h, double sound_speed, double alpha,
+                      double beta, double epsilon, double delta, const void* mass, const void* v,
+                      const void* pressure, void* dv, int accumulate);
 
 /* ------------------------------------------------------------------------------------
  * PrecomputedNeighborhoodSearch lists as CSR (src/nhs_precomputed.jl:187-206):
@@ -356,6 +376,15 @@ int pn_list_sph_density_periodic(const int32_t* row_ptr, const int32_t* col_idx,
                                  const void* mass, int64_t sel_first, int64_t sel_count,
                                  const int32_t* sel_list, void* rho, int accumulate,
                                  double search_radius, const double* box_size, void* stream);
+
+/* The n-body closure (benchmarks/n_body.jl:34-51) over precomputed lists
+ * (src/nhs_precomputed.jl:210-247: stored neighbours, distance recomputed, no radius test):
+ * dv[:, i] (+)= sum over the stored j with d >= sqrt(eps) of -G mass[j] pos_diff / d^3.
+ * x, y, dv: D x N of `dtype`; lists as pn_list_sph_density.  Enqueue only. */
+int pn_list_nbody_accel(const int32_t* row_ptr, const int32_t* col_idx, int64_t n_rows, int stored_base,
+                        int ndims, int dtype, double G, const void* x, const void* y, const void* mass,
+                        int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* dv,
+                        int accumulate, void* stream);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,13 @@ def lib():
         L.orc_threaded_wcsph_f32.restype = ci
         L.orc_threaded_wcsph_f32.argtypes = [vp, vp, i64, ci, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                              ctypes.c_float, ctypes.c_float, vp, vp, vp, vp]
+        L.orc_threaded_sph_rows_f32.restype = ci
+        L.orc_threaded_sph_rows_f32.argtypes = [vp, vp, i64, vp, i64, ci, ci, dbl, vp, vp, vp, vp, vp, vp]
+        L.orc_nbody.restype = None
+        L.orc_nbody.argtypes = [ci, dbl, dbl, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_wcsph_interact.restype = None
+        L.orc_wcsph_interact.argtypes = [ci, ci, dbl, dbl, dbl, dbl, dbl, dbl, dbl, vp, i64, vp, vp, vp, vp, vp,
+                                         vp, vp]
         L.orc_sph_density.restype = None
         L.orc_sph_density.argtypes = [ci, ci, dbl, vp, i64, vp, vp, vp, vp, vp]
         L.orc_tlsph_grad.restype = None
@@ -252,6 +259,23 @@ class Grid:
                                           _ptr(rho), _ptr(dv), ctypes.byref(tb), ctypes.byref(tp))
         return st, rho, dv, tb.value, tp.value
 
+    def threaded_sph_rows_f32(self, y_nd, rows, nthreads, mass, h, kernel="cubic_spline", P=None):
+        """Density (and, with P, pressure acceleration) in double over the grid's neighbour
+        sets for the query rows `rows` of y (the grid must be built from y).  Returns
+        (status, rho, rho_scale, dv, dv_scale), scales = sums of |term|."""
+        y = self._pts(y_nd)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        m = np.ascontiguousarray(np.broadcast_to(mass, (y.shape[0],)), dtype=np.float64)
+        Pd = None if P is None else np.ascontiguousarray(P, dtype=np.float64)
+        nr = len(rows)
+        rho, rs = np.zeros(nr), np.zeros(nr)
+        dv, ds = np.zeros((nr, 3)), np.zeros(nr)
+        st = lib().orc_threaded_sph_rows_f32(self.h, _ptr(y), y.shape[0], _ptr(rows), nr, int(nthreads),
+                                             KERNELS[kernel], float(h), _ptr(m),
+                                             _ptr(Pd) if Pd is not None else None, _ptr(rho), _ptr(rs),
+                                             _ptr(dv), _ptr(ds))
+        return st, rho, rs, (dv if P is not None else None), (ds if P is not None else None)
+
     def threaded_count_f32(self, y_nd, nthreads):
         y = self._pts(y_nd)
         counts = np.zeros(y.shape[0], dtype=np.int32)
@@ -259,6 +283,14 @@ class Grid:
         st = lib().orc_threaded_count_f32(self.h, _ptr(y), y.shape[0], int(nthreads), _ptr(counts),
                                           ctypes.byref(tb), ctypes.byref(tt))
         return st, counts, tb.value, tt.value
+
+
+def host_threads() -> int:
+    """Threads for the oracle's threaded paths: the host cores this process may use."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
 
 
 def trivial(x_nd, y_nd, radius, coord_dtype=np.float64, radius_dtype=np.float64, lists=False):
@@ -403,6 +435,40 @@ def loss_dual(which, x_nd, lists, k, dim, *, mass=None, kernel="cubic_spline", h
                         _ptr(rp), _ptr(cols), _ptr(m), float(rho0), float(c), float(gamma), int(k), int(dim),
                         ctypes.byref(v), ctypes.byref(d))
     return v.value, d.value
+
+
+SQRT_EPS = {np.dtype(np.float32): float(np.sqrt(np.finfo(np.float32).eps)),
+            np.dtype(np.float64): float(np.sqrt(np.finfo(np.float64).eps))}
+
+
+def nbody(x_nd, lists, mass, G, eltype=np.float64):
+    """benchmarks/n_body.jl:34-51 over CSR lists in double: (dv (N, D), scale)."""
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    dv = np.zeros_like(x)
+    scale = np.zeros(x.shape[0])
+    lib().orc_nbody(x.shape[1], float(G), SQRT_EPS[np.dtype(eltype)], _ptr(x), x.shape[0], _ptr(rp), _ptr(cols),
+                    _ptr(m), _ptr(dv), _ptr(scale))
+    return dv, scale
+
+
+def wcsph_interact(x_nd, lists, mass, v, p, h, c, *, kernel="wendland_c2", alpha=0.02, beta=0.0, epsilon=0.01,
+                   delta=0.1, eltype=np.float64):
+    """TrixiParticles' fluid-fluid interact! (benchmark_wcsph) over CSR lists in double.
+    v: (N, D+1) velocity then density.  Returns (dv (N, D+1), scale (N, 2))."""
+    x = np.ascontiguousarray(x_nd, dtype=np.float64)
+    rp, cols = _csr(lists)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    n, nd = x.shape
+    dv = np.zeros((n, nd + 1))
+    scale = np.zeros((n, 2))
+    lib().orc_wcsph_interact(nd, KERNELS[kernel], float(h), float(c), float(alpha), float(beta), float(epsilon),
+                             float(delta), SQRT_EPS[np.dtype(eltype)], _ptr(x), n, _ptr(rp), _ptr(cols), _ptr(m),
+                             _ptr(vv), _ptr(pp), _ptr(dv), _ptr(scale))
+    return dv, scale
 
 
 def tait_eos(rho, rho0, c, gamma=7.0):

@@ -822,6 +822,100 @@ void orc_sph_pressure_accel(int nd, int kernel, double h, const double* x, int64
     p.pressure(Kern<double>{kernel, nd, h}, mass, P, dv, scale);
 }
 
+// n-body acceleration of benchmarks/n_body.jl:34-51 over CSR lists, in double:
+//     dv_i = sum_{j in list(i), d >= sqrt_eps} -G m_j (x_i - x_j) / d^3,
+// scale_i = sum of |term| over the components (the summation-error scale).
+void orc_nbody(int nd, double G, double sqrt_eps, const double* x, int64_t n, const int64_t* rp,
+               const int32_t* cols, const double* mass, double* dv, double* scale) {
+    for (int64_t i = 0; i < n; ++i) {
+        double acc[3] = {0.0, 0.0, 0.0}, a = 0.0;
+        for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+            const int32_t j = cols[e];
+            double pd[3], d2 = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                pd[d] = x[nd * i + d] - x[nd * (int64_t)j + d];
+                d2 += pd[d] * pd[d];
+            }
+            const double dist = std::sqrt(d2);
+            if (dist < sqrt_eps) continue;  // `distance < sqrt(eps(ELTYPE)) && return`
+            for (int d = 0; d < nd; ++d) {
+                const double t = -G * mass[j] * pd[d] / (dist * dist * dist);
+                acc[d] += t;
+                a += std::fabs(t);
+            }
+        }
+        for (int d = 0; d < nd; ++d) dv[nd * i + d] = acc[d];
+        scale[i] = a;
+    }
+}
+
+// The fluid-fluid interaction of TrixiParticles 0.5 timed by the reference's benchmark_wcsph
+// (benchmarks/smoothed_particle_hydrodynamics.jl:44-102), restated term by term in double over
+// CSR lists.  v and dv are (nd+1) values per point: velocity, then density (ContinuityDensity).
+// scale[2 i] / scale[2 i + 1]: sums of |term| of the momentum / density equation of point i.
+void orc_wcsph_interact(int nd, int kernel, double h, double c, double alpha, double beta, double eps_visc,
+                        double delta, double sqrt_eps, const double* x, int64_t n, const int64_t* rp,
+                        const int32_t* cols, const double* mass, const double* v, const double* p, double* dv,
+                        double* scale) {
+    const Kern<double> k{kernel, nd, h};
+    const int nv = nd + 1;
+    for (int64_t a = 0; a < n; ++a) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0}, sm = 0.0, sd = 0.0;
+        const double rho_a = v[nv * a + nd];
+        for (int64_t e = rp[a]; e < rp[a + 1]; ++e) {
+            const int64_t b = cols[e];
+            const double rho_b = v[nv * b + nd];
+            double pd[3], d2 = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                pd[d] = x[nd * a + d] - x[nd * b + d];
+                d2 += pd[d] * pd[d];
+            }
+            const double dist = std::sqrt(d2);
+            // smoothing_kernel_grad: W'(d) / d * pos_diff, zero below sqrt(eps)
+            double grad[3] = {0.0, 0.0, 0.0};
+            if (dist >= sqrt_eps)
+                for (int d = 0; d < nd; ++d) grad[d] = k.dw(dist) / dist * pd[d];
+            const double m_b = mass[b];
+            // pressure acceleration (ContinuityDensity form)
+            const double pa_coef = -m_b * (p[a] + p[b]) / (rho_a * rho_b);
+            // Monaghan viscosity: approaching pairs only
+            double vdiff[3], vr = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                vdiff[d] = v[nv * a + d] - v[nv * b + d];
+                vr += vdiff[d] * pd[d];
+            }
+            double pi_ab = 0.0;
+            if (vr < 0.0) {
+                const double mu = h * vr / (d2 + eps_visc * h * h);
+                const double rho_mean = 0.5 * (rho_a + rho_b);
+                pi_ab = (-alpha * c * mu + beta * mu * mu) / rho_mean;
+            }
+            for (int d = 0; d < nd; ++d) {
+                const double tp = pa_coef * grad[d];
+                const double tv = -m_b * pi_ab * grad[d];
+                acc[d] += tp + tv;
+                sm += std::fabs(tp) + std::fabs(tv);
+            }
+            // continuity equation
+            double vg = 0.0;
+            for (int d = 0; d < nd; ++d) vg += vdiff[d] * grad[d];
+            const double tc = rho_a / rho_b * m_b * vg;
+            // Molteni-Colagrossi density diffusion (zero at coincident points)
+            double td = 0.0;
+            if (dist >= sqrt_eps) {
+                double psig = 0.0;
+                for (int d = 0; d < nd; ++d) psig += 2.0 * (rho_a - rho_b) * pd[d] / d2 * grad[d];
+                td = delta * h * c * psig * (m_b / rho_b);
+            }
+            acc[nd] += tc + td;
+            sd += std::fabs(tc) + std::fabs(td);
+        }
+        for (int d = 0; d < nv; ++d) dv[nv * a + d] = acc[d];
+        scale[2 * a] = sm;
+        scale[2 * a + 1] = sd;
+    }
+}
+
 void orc_pair_energy(int nd, double eps, double rc, const double* x, int64_t n, const int64_t* rp,
                      const int32_t* cols, double* e) {
     Phys<double> p{nd, x, n, rp, cols};
@@ -1072,6 +1166,67 @@ int orc_threaded_wcsph_f32(void* h, const float* y, int64_t n, int nthreads, flo
     if (t_build) *t_build = tb;
     if (t_phys) *t_phys = t1 - t0;
     return OK;
+}
+
+
+// Density and pressure acceleration in double over the grid's neighbour sets (the exact
+// Float32 predicate of traverse_point, src/nhs_grid.jl:519-575) for the query rows
+// rows[0 .. nrows), threaded over rows.  The grid must have been built from y (for example
+// by orc_threaded_count_f32).  Same sums as orc_sph_density / orc_sph_pressure_accel over
+// CSR lists, without materialising the lists, so very large clouds can be checked.
+// rho, rho_scale: nrows; dv: nrows x 3, dv_scale: nrows (sums of |term|).  P nullable.
+int orc_threaded_sph_rows_f32(void* h, const float* y, int64_t n, const int64_t* rows, int64_t nrows,
+                              int nthreads, int kernel, double hsml, const double* mass, const double* P,
+                              double* rho, double* rho_scale, double* dv, double* dv_scale) {
+    OracleGrid& og = *(OracleGrid*)h;
+    if (og.tc != 0 || og.tr != 0 || og.g.ndims != 3 || !og.built) return ERR_ARG;
+    const Geom& g = og.g;
+    const float r = (float)og.radius;
+    const Kern<double> k{kernel, 3, hsml};
+    std::atomic<int> status{OK};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) {
+        const int64_t b = nrows * t / nthreads, e = nrows * (t + 1) / nthreads;
+        th.emplace_back([&, b, e] {
+            for (int64_t q = b; q < e; ++q) {
+                const int64_t i = rows[q];
+                if (i < 0 || i >= n) {
+                    status.store(ERR_ARG);
+                    return;
+                }
+                double s = 0.0, sa = 0.0, a[3] = {0.0, 0.0, 0.0}, aa = 0.0;
+                auto body = [&](int32_t j, const float*, float) {
+                    double pd[3], d2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        pd[d] = (double)y[3 * i + d] - (double)y[3 * (int64_t)j + d];
+                        d2 += pd[d] * pd[d];
+                    }
+                    const double dist = std::sqrt(d2);
+                    const double w = mass[j] * k.w(dist);
+                    s += w;
+                    sa += std::fabs(w);
+                    if (P && d2 > 0.0) {
+                        const double c = -mass[j] * (P[i] + P[j]) * k.dw(dist) / dist;
+                        for (int d = 0; d < 3; ++d) {
+                            a[d] += c * pd[d];
+                            aa += std::fabs(c * pd[d]);
+                        }
+                    }
+                };
+                const int st = og.tcell == 0 ? traverse_point<float, float, float>(g, og.cs, y, y + 3 * i, r, body)
+                                             : traverse_point<float, float, double>(g, og.cs, y, y + 3 * i, r, body);
+                if (st != OK) status.store(st);
+                rho[q] = s;
+                if (rho_scale) rho_scale[q] = sa;
+                if (P) {
+                    for (int d = 0; d < 3; ++d) dv[3 * q + d] = a[d];
+                    if (dv_scale) dv_scale[q] = aa;
+                }
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    return status.load();
 }
 
 }  // extern "C"

@@ -4,8 +4,8 @@
 //      k_bin       cell key per point + bounds check + wave-aggregated histogram
 //      k_scan_*    exclusive scan of the histogram -> cell_start, non-empty cell list
 //      k_scatter   point -> provisional slot in its cell segment
-//      k_canon     deterministic order inside each cell (original index ascending)
-//                  and the cell-sorted coordinate planes
+//      k_canon     deterministic order inside each cell (original index ascending),
+//                  one thread per sorted slot, and the cell-sorted coordinate planes
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,6 +54,21 @@ __device__ __forceinline__ int cloud_of_cell(const Geom& g, int c) {
         else hi = mid;
     }
     return lo;
+}
+
+// Cells per dimension (x, y) of the grid holding cell c: the batch entry of its cloud, or
+// gn0, gn1 (the single grid's, read once from the kernel arguments by the caller).  The
+// entry is read as one 8-byte load, so the compiler cannot fold the two sources into one
+// load through a private copy of the kernel arguments.
+__device__ __forceinline__ void grid_dims_xy(const Geom& g, int c, int gn0, int gn1, int& n0, int& n1) {
+    if (g.table) {
+        const int2 e = *(const int2*)&g.table[cloud_of_cell(g, c)].n[0];
+        n0 = e.x;
+        n1 = e.y;
+    } else {
+        n0 = gn0;
+        n1 = gn1;
+    }
 }
 
 // Device-side status of one binning pass.
@@ -391,48 +406,59 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t
     }
 }
 
+// Provisional placement: selection position a goes to slot start[key] + slots[a] of its
+// cell segment (atomic order).  The slot holds a, not the point index, so k_canon can
+// recover the key and break ties between repeated selections.
 static __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ keys,
-                                                 const int32_t* __restrict__ slots, int64_t sel_first,
-                                                 const int32_t* __restrict__ sel_list, int64_t n_sel,
+                                                 const int32_t* __restrict__ slots, int64_t n_sel,
                                                  const int32_t* __restrict__ start,
                                                  int32_t* __restrict__ perm_tmp) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_sel) return;
     const int key = keys[a];
     if (key < 0) return;
-    perm_tmp[start[key] + slots[a]] = (int32_t)sel_point(a, sel_first, sel_list);
+    perm_tmp[start[key] + slots[a]] = (int32_t)a;
 }
 
 // Cells with more points than this keep the provisional (atomic) order instead of the
 // O(n^2) rank: their neighbour sets are unaffected, only the order inside the cell.
 constexpr int CANON_RANK_LIMIT = 1024;
 
-// Final position of each point: its cell's start plus the number of points of the same
-// cell with a smaller original index (canonical = 1), or its provisional atomic slot
+// Output-major: one thread per sorted slot k of the binned points.  The final position of
+// the point in slot k is its cell's start plus its rank in the cell segment by (original
+// index, selection position) ascending (canonical = 1: the SerialUpdate insertion order,
+// and repeated selections of one point get distinct consecutive slots), or k itself
 // (canonical = 0: the order of the reference's ParallelUpdate, nondeterministic inside a
-// cell; neighbour sets are the same).  Writes the permutation and the cell-sorted
-// coordinate planes (plane d at coords + d * cap).
+// cell; neighbour sets are the same).  A wavefront's stores land inside the cell segments
+// of its 64 slots, so the permutation and the cell-sorted coordinate planes (plane d at
+// coords + d * cap) are written in whole lines.
 template <int D, class TC>
-__global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
-                                               const int32_t* __restrict__ slots, int64_t sel_first,
-                                               const int32_t* __restrict__ sel_list, int64_t n_sel,
+__global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys, int64_t sel_first,
+                                               const int32_t* __restrict__ sel_list, const DevStatus* st,
                                                const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ perm_tmp,
                                                const TC* __restrict__ y, int32_t* __restrict__ sorted_idx,
                                                TC* __restrict__ coords, int64_t cap, int canonical) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_sel) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= st->n_binned) return;
+    const int32_t a = perm_tmp[k];
     const int key = keys[a];
-    if (key < 0) return;
     const int32_t p = (int32_t)sel_point(a, sel_first, sel_list);
     const int s = start[key], e = start[key + 1];
-    int pos;
+    int pos = (int)k;
     if (canonical && e - s <= CANON_RANK_LIMIT) {
         int rank = 0;
-        for (int q = s; q < e; ++q) rank += perm_tmp[q] < p;
+        if (!sel_list) {
+            // original index = sel_first + selection position: rank by position
+            for (int q = s; q < e; ++q) rank += perm_tmp[q] < a;
+        } else {
+            for (int q = s; q < e; ++q) {
+                const int32_t aq = perm_tmp[q];
+                const int32_t pq = sel_list[aq];
+                rank += (pq < p) || (pq == p && aq < a);
+            }
+        }
         pos = s + rank;
-    } else {
-        pos = s + slots[a];
     }
     sorted_idx[pos] = p;
 #pragma unroll

@@ -14,10 +14,9 @@
 //   * Every other functor: the accepted (query, candidate) pairs are appended to a pair
 //     list in LDS in query-major order (per query in the reference's candidate order:
 //     stencil rows first-dimension fastest, then the in-cell order).  When the list is
-//     full, or the candidate chunk ends, the 64 lanes split it into 64 contiguous ranges
-//     of equal length and run the pair body.  A query whose pairs span several lanes is
-//     summed by the lane holding its first pair, adding the other lanes' partial sums in
-//     lane order, so sums are deterministic.
+//     full, or the candidate chunk ends, every query of the list gets the same number of
+//     lanes, which run the pair body over its pairs round-robin; their partial sums are
+//     combined by a tree of fixed order, so sums are deterministic.
 #pragma once
 
 #include "pn_traverse.h"
@@ -54,6 +53,28 @@ __device__ __forceinline__ T lane_bcast(T v, int i) {
         const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), i);
         return __builtin_bit_cast(T, (long long)(((unsigned long long)hi << 32) | lo));
     }
+}
+
+// v with lane `lane` set to the wave-uniform value `val` (v_writelane_b32: one VALU
+// instruction; no builtin for it here).
+__device__ __forceinline__ int write_lane(int v, int val, int lane) {
+    // the lane select goes through m0: one instruction may read only one other SGPR
+    asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(val), "{m0}"(lane));
+    return v;
+}
+
+// v of lane src (mod 64), for any trivially copyable accumulator (word by word).
+template <class A>
+__device__ __forceinline__ A shfl_acc(const A& v, int src) {
+    static_assert(sizeof(A) % 4 == 0, "accumulators are whole words");
+    constexpr int W = sizeof(A) / 4;
+    struct Words {
+        int w[W];
+    };
+    Words in = __builtin_bit_cast(Words, v), out;
+#pragma unroll
+    for (int k = 0; k < W; ++k) out.w[k] = __shfl(in.w[k], src & (WAVE - 1), WAVE);
+    return __builtin_bit_cast(A, out);
 }
 
 // Candidate groups held in registers per chunk.  Counting: 6 (a 3D stencil of ~14 points per
@@ -110,13 +131,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];
     PairT* const s_pairs = (PairT*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
+    // the query batch's coordinates: read with one broadcast LDS load per query in the test
+    // loop and per lane in the pair bodies
+    __shared__ Pack<TC, D> s_qc[WAVE];
     // Per-query state (f.query): kept in LDS when it is one word or the CSR fill's row bounds
     // (read at almost every pair), else reloaded from global memory where a flush needs it
     // (larger states would cost occupancy).
     constexpr bool kQstLds = sizeof(QS) <= 4 || F::kFill;
     __shared__ QS s_qst[kQstLds ? LQ : 1];
     __shared__ Acc s_acc[LQ];
-    __shared__ Acc s_con[LQ];
     __shared__ int s_written[F::kFill ? WAVE : 1];
 
     const int lane = threadIdx.x;
@@ -163,19 +186,13 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     bool cell_ok = true;
 
     // Runs the pair body over pairs [0, P) of list `prp`, which hold queries i0 .. i1-1
-    // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Lane t takes pairs
-    // [t*P/64, (t+1)*P/64).  A query inside one lane's range is added to s_acc by that
-    // lane; a query spanning lanes is summed by the lane holding its first pair, which
-    // adds the following lanes' partial sums (s_con) in lane order.  The pair bodies read a
-    // query's coordinates from its sorted plane (L1/L2-resident; keeping them out of LDS buys
-    // occupancy).
+    // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Each query gets
+    // L = 64 / (i1 - i0) lanes, which take its pairs round-robin; the L partial sums are
+    // combined by a tree of fixed order and added to s_acc by the query's first lane, so sums
+    // are deterministic.  The pair bodies read a query's coordinates from its sorted plane
+    // (L1/L2-resident; keeping them out of LDS buys occupancy).
     int cur_q0 = 0;  // sorted position of the current query batch
-    auto qpack = [&](int q) {
-        Pack<TC, D> pk;
-#pragma unroll
-        for (int d = 0; d < D; ++d) pk.v[d] = ((const TC*)Q.coords)[(int64_t)d * Q.cap + cur_q0 + q];
-        return pk;
-    };
+    auto qpack = [&](int q) { return s_qc[q]; };
     // ... and its per-query state.
     auto qstate = [&](int q) -> QS {
         if constexpr (kQstLds) return s_qst[q];
@@ -183,7 +200,6 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     };
     auto flush = [&](int i0, int i1, int P, int batch, int chunk, const uint16_t* qsp, const PairT* prp) {
         const int nf = i1 - i0;
-        s_con[lane] = f.init();
         __syncthreads();
         if constexpr (F::kCache && PCM == 1) {
             if (cell_ok && P > 0) {
@@ -226,45 +242,18 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             __syncthreads();
             return;
         }
-        const int kb = (lane * P) >> 6, ke = ((lane + 1) * P) >> 6;
-        int own = -1;
-        Acc own_acc = f.init();
-        if (kb < ke) {
-            int lo = 0, hi = nf;  // largest fq with qsp[fq] <= kb
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (qsp[mid] <= kb) lo = mid;
-                else hi = mid;
-            }
-            int fq = lo;
-            int next = qsp[fq + 1];
-            Pack<TC, D> qp = qpack(i0 + fq);
-            QS qst = qstate(i0 + fq);
-            Acc a = f.init();
-            auto finish = [&]() {
-                const bool starts = qsp[fq] >= kb;
-                if (starts && next <= ke) {
-                    Acc t = s_acc[i0 + fq];
-                    f.combine(t, a);
-                    s_acc[i0 + fq] = t;
-                } else if (starts) {
-                    own = fq;
-                    own_acc = a;
-                } else {
-                    s_con[lane] = a;
-                }
-            };
-            for (int k = kb; k < ke; ++k) {
-                if (k >= next) {
-                    finish();
-                    do {
-                        ++fq;
-                        next = qsp[fq + 1];
-                    } while (k >= next);
-                    qp = qpack(i0 + fq);
-                    qst = qstate(i0 + fq);
-                    a = f.init();
-                }
+        // L lanes per query (the largest power of two with L * nf <= 64): lane l takes query
+        // q = l / L and its pairs qsp[q] + (l % L) + L t, so no lane changes query inside the
+        // loop; the L partial sums of a query are then combined by a butterfly of fixed order.
+        const int L = WAVE / max(nf, 1);
+        // lane / L exactly: (lane + 1/2) / L is at least 1/(2L) away from an integer
+        const int q = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)L)), sub = lane - q * L;
+        Acc a = f.init();
+        if (q < nf) {
+            const Pack<TC, D> qp = qpack(i0 + q);
+            const QS qst = qstate(i0 + q);
+            const int ke = qsp[q + 1];
+            for (int k = qsp[q] + sub; k < ke; k += L) {
                 const int m = prp[k];
                 const Pack<TC, D> pk = s_c[m];
                 TR pd[D];
@@ -277,27 +266,33 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     d2 = d2 + p2;
                 }
                 if constexpr (PER) d2 = periodic_fix(pd, d2);
-                {
-                    int j = 0;
-                    if constexpr (F::kIndex) j = s_j[m];
-                    TR fj[NF > 0 ? NF : 1];
-                    if constexpr (kPackF) fj[0] = pk.v[3];
-                    if constexpr (NFL > 0) {
+                int j = 0;
+                if constexpr (F::kIndex) j = s_j[m];
+                TR fj[NF > 0 ? NF : 1];
+                if constexpr (kPackF) fj[0] = pk.v[3];
+                if constexpr (NFL > 0) {
 #pragma unroll
-                        for (int fi = FOFF; fi < NF; ++fi) fj[fi] = s_f[fi - FOFF][m];
-                    }
-                    f.template pair<TR, D>(a, qst, pd, d2, j, fj, nullptr);
+                    for (int fi = FOFF; fi < NF; ++fi) fj[fi] = s_f[fi - FOFF][m];
                 }
+                f.template pair<TR, D>(a, qst, pd, d2, j, fj, nullptr);
             }
-            finish();
         }
-        __syncthreads();
-        if (own >= 0) {
-            const int e = qsp[own + 1];
-            for (int t = lane + 1; t < WAVE && ((t * P) >> 6) < e; ++t) f.combine(own_acc, s_con[t]);
-            Acc t = s_acc[i0 + own];
-            f.combine(t, own_acc);
-            s_acc[i0 + own] = t;
+        // tree over the query's L lanes towards its first lane: fold the lanes past the largest
+        // power of two P <= L, then halve
+        int P2 = 1;
+        while (2 * P2 <= L) P2 *= 2;
+        if (L > P2) {
+            const Acc t = shfl_acc(a, lane + P2);
+            if (sub < L - P2) f.combine(a, t);
+        }
+        for (int o = P2 / 2; o > 0; o >>= 1) {
+            const Acc t = shfl_acc(a, lane + o);
+            if (sub < o) f.combine(a, t);
+        }
+        if (q < nf && sub == 0) {
+            Acc t = s_acc[i0 + q];
+            f.combine(t, a);
+            s_acc[i0 + q] = t;
         }
         __syncthreads();
     };
@@ -309,6 +304,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     // Persistent grid: XCD x (= blockIdx % 8) walks a contiguous range of the non-empty
     // cells, so the cells that share stencil rows run on one XCD's L2.
     const int n_work = Q.st->n_nonempty;
+    const int gn0 = g.n[0], gn1 = g.n[1];
     const int xcd = blockIdx.x & 7, per_xcd = nblocks >> 3;
     const int wchunk = (n_work + 7) >> 3;
     const int w_end = min(n_work, (xcd + 1) * wchunk);
@@ -318,12 +314,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         cell_ok = true;
         const int qc = Q.nonempty[w];
         const int qs = Q.start[qc], qe = Q.start[qc + 1];
-        int n0 = g.n[0], n1 = g.n[1];
-        if (g.table) {
-            const GeomEntry& e = g.table[cloud_of_cell(g, qc)];
-            n0 = e.n[0];
-            n1 = e.n[1];
-        }
+        int n0, n1;
+        grid_dims_xy(g, qc, gn0, gn1, n0, n1);
         // Stencil rows, each one or two pieces of the sorted order: piece u covers
         // [rdel[u] + pre[u], rdel[u] + pre[u + 1]).  A row of 3 x-adjacent cells is one
         // contiguous range; with a periodic box a row whose x-neighbour wraps is two
@@ -468,8 +460,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
 #pragma unroll
             for (int d = 0; d < D; ++d) xq[d] = qact ? qcrd[(int64_t)d * Q.cap + q0 + lane] : (TC)0;
             orig_i = qact ? Q.sorted_idx[q0 + lane] : 0;
+            __syncthreads();  // the previous batch is done with s_qc
+            {
+                Pack<TC, D> pk;
+#pragma unroll
+                for (int d = 0; d < (D == 3 ? 4 : D); ++d) pk.v[d] = d < D ? xq[d] : (TC)0;
+                s_qc[lane] = pk;
+            }
+            __syncthreads();
             if constexpr (!kLight) {
-                __syncthreads();
                 if (qact) {
                     if constexpr (kQstLds) s_qst[lane] = f.query(orig_i);
                     s_acc[lane] = f.init();
@@ -557,12 +556,14 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                         for (int i = 0; i < nq; ++i) {
                             TC xs[D];
 #pragma unroll
-                            for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+                            for (int d = 0; d < D; ++d) xs[d] = s_qc[i].v[d];
                             int c = 0;
 #pragma unroll
                             for (int gi = 0; gi < NG; ++gi)
                                 c += __builtin_popcountll(__builtin_amdgcn_ballot_w64(test(xs, gi)));
-                            mycount += lane == i ? c : 0;
+                            // lane i holds query i's count (no earlier chunk: nothing to add)
+                            if (chunk != 0) c += __builtin_amdgcn_readlane(mycount, i);
+                            mycount = write_lane(mycount, c, i);
                         }
                     });
                 } else {
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                             if (lane == 0) s_qstart[i - i0] = (uint16_t)P;
                             TC xs[D];
 #pragma unroll
-                            for (int d = 0; d < D; ++d) xs[d] = lane_bcast(xq[d], i);
+                            for (int d = 0; d < D; ++d) xs[d] = s_qc[i].v[d];
 #pragma unroll
                             for (int gi = 0; gi < NG; ++gi) {
                                 // accepted lanes take the next list slots in lane order (the

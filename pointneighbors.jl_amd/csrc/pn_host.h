@@ -277,11 +277,12 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
         if (rc2) return rc2;
         if (sel_count > 0) {
             hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
-                               s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_first, sel_list, sel_count,
-                               s.start.as<int32_t>(), s.perm_tmp.as<int32_t>());
+                               s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_count, s.start.as<int32_t>(),
+                               s.perm_tmp.as<int32_t>());
             PN_CHECK_LAUNCH();
+            // one thread per binned slot (at most sel_count; the count is read on the device)
             hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
-                               s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_first, sel_list, sel_count,
+                               s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
                                s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
                                s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order);
             PN_CHECK_LAUNCH();

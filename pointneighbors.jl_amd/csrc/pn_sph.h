@@ -39,6 +39,17 @@ __device__ __forceinline__ double psqrt(double x) { return sqrt(x); }
 // 1/sqrt(x) for x > 0, 0 for x == 0
 __device__ __forceinline__ float prsqrt0(float x) { return x > 0.0f ? __builtin_amdgcn_rsqf(x) : 0.0f; }
 __device__ __forceinline__ double prsqrt0(double x) { return x > 0.0 ? 1.0 / sqrt(x) : 0.0; }
+// sqrt(eps(T)) in T, Julia's threshold below which a distance counts as zero
+template <class T>
+__device__ __forceinline__ T sqrt_eps();
+template <>
+__device__ __forceinline__ float sqrt_eps<float>() {
+    return 3.4526698e-4f;  // sqrt(2^-23)
+}
+template <>
+__device__ __forceinline__ double sqrt_eps<double>() {
+    return 1.4901161193847656e-8;  // sqrt(2^-52) = 2^-26
+}
 __device__ __forceinline__ float pfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double pfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -62,14 +73,16 @@ struct SmoothingKernel {
 
     __device__ __forceinline__ TR sqrt_(TR x) const { return psqrt(x); }
 
-    // W(r), W'(r) = dW/dr, W''(r); branch-free in q (both pieces evaluated, selected)
+    // W(r), W'(r) = dW/dr, W''(r) without branches.  The cubic spline is written with the
+    // truncated powers of its B-spline form, a = max(2 - q, 0), b = max(1 - q, 0):
+    //     W = s (a^3 - 4 b^3) / 4,  dW/dq = s (3 b^2 - 3/4 a^2),  d2W/dq2 = s (3/2 a - 6 b),
+    // which equal the piecewise polynomials on q < 1, 1 <= q < 2 and q >= 2.
     __device__ __forceinline__ TR w(TR r) const {
         const TR q = r * inv_h;
         if (kind == KERNEL_CUBIC_SPLINE) {
-            const TR inner = pfma(q * q, pfma((TR)0.75, q, (TR)-1.5), (TR)1);
-            const TR t = (TR)2 - q;
-            const TR outer = (TR)0.25 * t * t * t;
-            return sigma * (q < (TR)1 ? inner : (q < (TR)2 ? outer : (TR)0));
+            const TR a = max((TR)2 - q, (TR)0), b = max((TR)1 - q, (TR)0);
+            const TR b3 = b * b * b;
+            return (TR)0.25 * sigma * pfma(a * a, a, (TR)-4 * b3);
         }
         const TR t = max((TR)1 - (TR)0.5 * q, (TR)0);
         const TR t2 = t * t;
@@ -80,10 +93,8 @@ struct SmoothingKernel {
         const TR q = r * inv_h;
         const TR s1 = sigma * inv_h;
         if (kind == KERNEL_CUBIC_SPLINE) {
-            const TR inner = q * pfma((TR)2.25, q, (TR)-3);
-            const TR t = (TR)2 - q;
-            const TR outer = (TR)-0.75 * t * t;
-            return s1 * (q < (TR)1 ? inner : (q < (TR)2 ? outer : (TR)0));
+            const TR a = max((TR)2 - q, (TR)0), b = max((TR)1 - q, (TR)0);
+            return s1 * pfma((TR)3 * b, b, (TR)-0.75 * a * a);
         }
         const TR t = max((TR)1 - (TR)0.5 * q, (TR)0);
         return s1 * ((TR)-5 * q * t * t * t);
@@ -93,9 +104,8 @@ struct SmoothingKernel {
         const TR q = r * inv_h;
         const TR s2 = sigma * inv_h * inv_h;
         if (kind == KERNEL_CUBIC_SPLINE) {
-            const TR inner = pfma((TR)4.5, q, (TR)-3);
-            const TR outer = (TR)1.5 * ((TR)2 - q);
-            return s2 * (q < (TR)1 ? inner : (q < (TR)2 ? outer : (TR)0));
+            const TR a = max((TR)2 - q, (TR)0), b = max((TR)1 - q, (TR)0);
+            return s2 * pfma((TR)1.5, a, (TR)-6 * b);
         }
         const TR t = max((TR)1 - (TR)0.5 * q, (TR)0);
         return s2 * (TR)5 * t * t * pfma((TR)2, q, (TR)-1);

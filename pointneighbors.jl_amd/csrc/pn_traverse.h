@@ -254,6 +254,7 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
     // Persistent grid: XCD x (= blockIdx % 8) walks a contiguous range of the non-empty
     // cells, so the cells that share stencil rows run on one XCD's L2.
     const int n_work = Q.st->n_nonempty;
+    const int gn0 = g.n[0], gn1 = g.n[1];
     const int xcd = blockIdx.x & 7, per_xcd = nblocks >> 3;
     const int chunk = (n_work + 7) >> 3;
     const int w_end = min(n_work, (xcd + 1) * chunk);
@@ -286,12 +287,8 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
                 hash_query<D, TC, TR, TCELL, F, HPER>(g, Q, G, r2, f, (int64_t)k, &s_cnt[F::kFill ? lane : 0]);
         }
     } else {
-    int n0 = g.n[0], n1 = g.n[1];
-    if (g.table) {
-        const GeomEntry& e = g.table[cloud_of_cell(g, qc)];
-        n0 = e.n[0];
-        n1 = e.n[1];
-    }
+    int n0, n1;
+    grid_dims_xy(g, qc, gn0, gn1, n0, n1);
 
     // stencil rows
     int rb = 0, rlen = 0;

@@ -9,7 +9,8 @@ GPU (strides (1, D), the memory of Julia's column-major matrix).
 
 from ._lib import ArgumentError, BoundsError, PointNeighborsError, load
 from .cell_lists import DynamicVectorOfVectors, FullGridCellList, SpatialHashingCellList
-from .neighborhood_search import (CountNeighbors, NeighborIndex, One, SumNeighborIndices,
+from .neighborhood_search import (CountNeighbors, Distance, DistanceSquared, NeighborIndex, One, PosDiff,
+                                  SumNeighborIndices, mapreduce_point_neighbors,
                                   copy_neighborhood_search, foreach_neighbor,
                                   foreach_neighbor_unsafe, foreach_point_neighbor,
                                   foreach_point_neighbor_unsafe, freeze_neighborhood_search,
@@ -19,12 +20,15 @@ from .nhs_grid import (GridNeighborhoodSearch, ParallelIncrementalUpdate, Parall
                        PeriodicBox, SemiParallelUpdate, SerialIncrementalUpdate, SerialUpdate,
                        initialize_grid, update_grid)
 from .nhs_precomputed import NeighborLists, PrecomputedNeighborhoodSearch
+from .interactions import NBodyAcceleration, WCSPHInteraction, state_equation_cole
 from .util import floor_to_int
 
 __all__ = [
     "ArgumentError", "BoundsError", "PointNeighborsError", "load",
     "DynamicVectorOfVectors", "FullGridCellList", "SpatialHashingCellList",
-    "CountNeighbors", "NeighborIndex", "One", "SumNeighborIndices",
+    "CountNeighbors", "Distance", "DistanceSquared", "NeighborIndex", "One", "PosDiff",
+    "SumNeighborIndices", "mapreduce_point_neighbors", "NBodyAcceleration", "WCSPHInteraction",
+    "state_equation_cole",
     "copy_neighborhood_search", "foreach_neighbor", "foreach_neighbor_unsafe",
     "foreach_point_neighbor", "foreach_point_neighbor_unsafe", "freeze_neighborhood_search",
     "initialize", "mapreduce_neighbor", "mapreduce_neighbor_unsafe", "requires_update",

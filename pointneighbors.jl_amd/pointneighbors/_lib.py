@@ -12,7 +12,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpointneighbors_hip.so")
+# the in-tree build; POINTNEIGHBORS_HIP_LIB names another build of the same library (A/B runs)
+LIB_PATH = os.environ.get("POINTNEIGHBORS_HIP_LIB") or os.path.join(os.path.dirname(_HERE),
+                                                                      "libpointneighbors_hip.so")
 
 PN_OK = 0
 PN_ERR_ARGUMENT = 1
@@ -98,6 +100,12 @@ SIGNATURES = {
                                            _vp]),
     "pn_tlsph_interaction_workspace_bytes": (_c_i64, [_c_int, _c_int, _c_i64]),
     "pn_tlsph_pk1_rho2": (_c_int, [_vp, _vp, _vp, _c_i64, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp]),
+    "pn_nbody_accel": (_c_int, [_vp, _c_int, _c_dbl, _vp, _vp, _c_int]),
+    "pn_list_nbody_accel": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _c_i64,
+                                     _c_i64, _vp, _vp, _c_int, _vp]),
+    "pn_wcsph_interact": (_c_int, [_vp, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _vp, _vp, _vp,
+                                   _vp, _c_int]),
+    "pn_mapreduce_neighbors": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "pn_list_sph_density_periodic": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                               _vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _c_int, _c_dbl, _p_dbl,
                                               _vp]),

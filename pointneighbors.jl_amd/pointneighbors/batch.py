@@ -124,3 +124,6 @@ def displace_uniform(x, amplitude, seed, step):
     check(load().pn_displace_uniform(ctypes.c_void_p(x.data_ptr()), x.numel(), code, float(amplitude),
                                      int(seed) & 0xFFFFFFFFFFFFFFFF, ctypes.c_void_p(step.data_ptr()),
                                      ctypes.c_void_p(_device.current_stream_handle(x.device))), "displace_uniform")
+    # an in-place write torch did not see: bump the version counter so a search that
+    # binned x re-reads its coordinates
+    torch.autograd.graph.increment_version(x)

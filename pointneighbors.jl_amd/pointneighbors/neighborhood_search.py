@@ -47,8 +47,26 @@ class One:
     """`(i, j, pos_diff, d) -> 1`."""
 
 
-MENU = ("CountNeighbors", "SumNeighborIndices", "SPHDensity", "SPHPressureForce",
-        "WendlandPairEnergy", "DensityLossGradient", "PairEnergyGradient")
+class Distance:
+    """`(i, j, pos_diff, d) -> d` (the traversal's distance, sqrt of d^2 in the radius type)."""
+
+
+class DistanceSquared:
+    """`(i, j, pos_diff, d) -> d^2` (dot(pos_diff, pos_diff) as the pair predicate computes it)."""
+
+
+class PosDiff:
+    """`(i, j, pos_diff, d) -> pos_diff[k]` (0-based component k)."""
+
+    def __init__(self, k):
+        self.k = int(k)
+
+
+MENU = ("CountNeighbors", "SumNeighborIndices", "SPHDensity", "SPHPressureAcceleration",
+        "WendlandPairEnergy", "NBodyAcceleration", "WCSPHInteraction")
+# value functions and operators of mapreduce_neighbor (pn_mapreduce_neighbors codes)
+_MR_FN = {One: 0, NeighborIndex: 1, Distance: 2, DistanceSquared: 3}
+_MR_OP = {operator.add: 0, min: 1, max: 2}
 
 
 def _menu_error(f):
@@ -112,33 +130,87 @@ def foreach_point_neighbor_unsafe(f, system_coords, neighbor_coords, neighborhoo
                                   parallelization_backend=parallelization_backend, points=points)
 
 
+def _mr_codes(f, op, ndims):
+    if isinstance(f, PosDiff):
+        if not 0 <= f.k < ndims:
+            raise ArgumentError(f"PosDiff({f.k}) is out of range for {ndims} dimensions")
+        fn = 4 + f.k
+    else:
+        fn = _MR_FN.get(f if isinstance(f, type) else type(f))
+    if fn is None:
+        raise ArgumentError(
+            f"{f!r} cannot run on the MI355X engine: mapreduce_neighbor takes one of the value "
+            "functions One, NeighborIndex, Distance, DistanceSquared, PosDiff(k); arbitrary "
+            "closures cannot cross the C-ABI")
+    if op not in _MR_OP:
+        raise ArgumentError("mapreduce_neighbor on the MI355X engine supports op = +, min, max")
+    return fn, _MR_OP[op]
+
+
+def mapreduce_point_neighbors(f, op, system_coords, neighbor_coords, neighborhood_search, *,
+                              init, points=None):
+    """mapreduce_neighbor (src/neighborhood_search.jl:340-348) for every selected point in one
+    launch: returns (values, counts), float64 and int32 device vectors over the columns of
+    system_coords.  values[i] = op(init, reduction over the neighbours of i), or init where i
+    has no neighbour (or is not selected); the reduction runs in double."""
+    import torch
+    from .nhs_grid import GridNeighborhoodSearch
+    nhs = neighborhood_search
+    if not isinstance(nhs, GridNeighborhoodSearch):
+        raise ArgumentError("mapreduce_point_neighbors runs on a GridNeighborhoodSearch")
+    fn, opc = _mr_codes(f, op, nhs.ndims)
+    mode, x = nhs._prepare(system_coords, neighbor_coords, points)
+    n = x.shape[1]
+    values = torch.full((n,), float("nan"), dtype=torch.float64, device=x.device)
+    counts = torch.zeros(n, dtype=torch.int32, device=x.device)
+    check(load().pn_mapreduce_neighbors(nhs._handle, mode, fn, opc, 0, ctypes.c_void_p(values.data_ptr()),
+                                        ctypes.c_void_p(counts.data_ptr())), "mapreduce_neighbor")
+    init_t = torch.full_like(values, float(init))
+    if op is operator.add:
+        red = init_t + values
+    elif op is min:
+        red = torch.minimum(init_t, values)
+    else:
+        red = torch.maximum(init_t, values)
+    return torch.where(counts > 0, red, init_t), counts
+
+
 def mapreduce_neighbor(f, op, system_coords, neighbor_coords, neighborhood_search, point, *,
                        init):
-    """src/neighborhood_search.jl:340-348 for one point, with f in (NeighborIndex, One)
-    and op = +.  Returns `init` for an empty neighbourhood without evaluating f or op
-    (test/neighborhood_search.jl:445-466)."""
+    """src/neighborhood_search.jl:340-348 for one point: op(init, f(neighbour 1), ...).  f is
+    one of One, NeighborIndex (0-based j), Distance, DistanceSquared, PosDiff(k); op is +, min
+    or max.  One launch, one host synchronization.  Returns `init` for an empty neighbourhood
+    without evaluating f or op (test/neighborhood_search.jl:445-466).  On a precomputed search
+    the stored lists give One and NeighborIndex with +."""
     import torch
-    if op is not operator.add:
-        raise ArgumentError("mapreduce_neighbor on the MI355X engine supports op = +")
+    from .nhs_precomputed import PrecomputedNeighborhoodSearch
+    nhs = neighborhood_search
     n = system_coords.shape[1]
-    dev = system_coords.device
-    if f is NeighborIndex or isinstance(f, NeighborIndex):
-        out = torch.zeros(n, dtype=torch.int64, device=dev)
+    if not 0 <= point < n:
+        raise BoundsError(f"attempt to access {n}-element columns at index {point}")
+    integral = (f in (One, NeighborIndex) or isinstance(f, (One, NeighborIndex))) and op is operator.add
+    if isinstance(nhs, PrecomputedNeighborhoodSearch):
+        _mr_codes(f, op, nhs.ndims)
+        if not integral:
+            raise ArgumentError("mapreduce_neighbor over precomputed lists supports One and "
+                                "NeighborIndex with op = +")
+        dev = system_coords.device
         cnt = torch.zeros(n, dtype=torch.int32, device=dev)
-        foreach_point_neighbor(SumNeighborIndices(out), system_coords, neighbor_coords,
-                               neighborhood_search, points=[point])
-        foreach_point_neighbor(CountNeighbors(cnt), system_coords, neighbor_coords,
-                               neighborhood_search, points=[point])
-        if int(cnt[point]) == 0:
-            return init
-        return init + int(out[point])
-    if f is One or isinstance(f, One):
-        cnt = torch.zeros(n, dtype=torch.int32, device=dev)
-        foreach_point_neighbor(CountNeighbors(cnt), system_coords, neighbor_coords,
-                               neighborhood_search, points=[point])
+        foreach_point_neighbor(CountNeighbors(cnt), system_coords, neighbor_coords, nhs, points=[point])
         c = int(cnt[point])
-        return init if c == 0 else init + c
-    _menu_error(f)
+        if c == 0:
+            return init
+        if f is One or isinstance(f, One):
+            return init + c
+        out = torch.zeros(n, dtype=torch.int64, device=dev)
+        foreach_point_neighbor(SumNeighborIndices(out), system_coords, neighbor_coords, nhs, points=[point])
+        return init + int(out[point])
+    values, counts = mapreduce_point_neighbors(f, op, system_coords, neighbor_coords, nhs, init=init,
+                                               points=[point])
+    v, c = values[point].item(), int(counts[point].item())
+    if c == 0:
+        return init
+    return int(round(v)) if integral and isinstance(init, int) else v
 
 
 mapreduce_neighbor_unsafe = mapreduce_neighbor

@@ -145,9 +145,13 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         self._coord_dtype = None
         self._built = False
         self._empty = False     # initialized with search_radius < eps()
-        self._y_key = None      # (data_ptr, shape, stride, version) of the last update
+        # Coordinate bookkeeping.  A key is (data_ptr, shape, stride, torch version counter).
+        self._y_key = None      # key of y as binned by the last update!; None once it was
+                                # declared changed (coordinates_changed)
+        self._planes_key = None  # key of the y the sorted coordinate planes now hold
+        self._y_ref = None      # the binned y: held so its memory cannot be handed to another tensor
+        self._stale = False     # coordinates_changed() was called since the planes were filled
         self._y_full = True     # eachindex_y covered all columns
-        self._refreshed = False
         self._n_y = 0
         self._tuning = {}
 
@@ -292,8 +296,10 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         del keep
         self._built = True
         self._y_key = (y.data_ptr(), tuple(y.shape), y.stride(), y._version)
+        self._planes_key = self._y_key
+        self._y_ref = y
+        self._stale = False
         self._y_full = _device.is_full(eachindex_y, n)
-        self._refreshed = False
         self._n_y = n
 
     def initialize(self, x, y, *, eachindex_y=None):
@@ -308,6 +314,14 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         if not points_moving[1]:
             return self
         self._build(y, eachindex_y, check_errors)
+        return self
+
+    def coordinates_changed(self):
+        """Declares that the binned y was written without torch seeing it (through a raw
+        pointer, another library or a HIP kernel of the caller's): the next traversal
+        re-reads the current coordinates, as the reference always does
+        (src/nhs_grid.jl:543-545).  Cell membership stays that of the last update!."""
+        self._stale = True
         return self
 
     def check(self):
@@ -340,8 +354,12 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
                                 "the neighborhood search was updated with")
         lib = load()
         self._set_stream(x)
+        # The sorted coordinate planes hold the y they were last filled from.  They are
+        # current while that is this y (the binned y is held in _y_ref, so its address cannot
+        # be reused), torch's version counter has not moved (every in-place torch op and every
+        # C-ABI writer of this package bumps it) and coordinates_changed() was not called.
         y_key = (y.data_ptr(), tuple(y.shape), y.stride(), y._version)
-        if y_key != self._y_key or self._refreshed:
+        if y_key != self._planes_key or self._stale:
             # cell membership from the last update!, coordinates read now
             # (src/nhs_grid.jl:543-545)
             if y.shape[1] != self._n_y:
@@ -349,9 +367,14 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
                                   "the last update!")
             check(lib.pn_grid_refresh(self._handle, ctypes.c_void_p(y.data_ptr()), y.shape[1]),
                   "foreach_point_neighbor")
-            self._refreshed = y_key != self._y_key
+            if self._stale:
+                self._y_key = None  # the binned tensor no longer holds what was binned
+            self._planes_key = y_key
+            self._stale = False
         nx = x.shape[1]
-        x_is_y = (_device.same_storage(x, y) and y_key == self._y_key and not self._refreshed)
+        # the grid's own cells serve as query cells only while y is exactly what was binned;
+        # otherwise the queries are binned from the current x (their cells follow x now)
+        x_is_y = _device.same_storage(x, y) and y_key == self._y_key
         if x_is_y and self._y_full and _device.is_full(points, nx):
             return PN_QUERIES_GRID, x
         first, count, ptr, keep = _device.selection(points, nx, x.device)

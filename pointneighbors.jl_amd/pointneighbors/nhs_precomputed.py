@@ -16,6 +16,10 @@ from .cell_lists import FullGridCellList
 from .nhs_grid import GridNeighborhoodSearch, _GridMeta
 from .util import FLOAT64_EPS, scalar_dtype
 
+# max_points_per_cell of an inner grid the search makes for itself: the grid's largest
+# accepted value (a 3^D stencil's candidate count must fit in int32), i.e. no capacity
+AUTO_GRID_CAPACITY = 2**31 // 27
+
 
 def max_neighbors_default(ndims):
     """src/nhs_precomputed.jl:114-124 (the code's values; the docstring says 324)."""
@@ -119,6 +123,7 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
         self.sort_neighbor_lists = sort_neighbor_lists
         self.neighbor_lists = _lists if _lists is not None else NeighborLists(max_neighbors=self.max_neighbors)
         self._frozen = False
+        self._auto = False  # the inner grid was made by this search (no user grid given)
 
     def requires_update(self):
         """src/nhs_precomputed.jl:128"""
@@ -146,9 +151,32 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
         else:
             import numpy as np
             lo = hi = np.zeros(self.ndims, dtype=_device.np_dtype(x))
-        cl = FullGridCellList(min_corner=lo - r, max_corner=hi + r, search_radius=r)
+        # no per-cell capacity (the reference's default inner search is an unbounded
+        # DictionaryCellList); the largest limit the grid accepts
+        cl = FullGridCellList(min_corner=lo - r, max_corner=hi + r, search_radius=r,
+                              max_points_per_cell=AUTO_GRID_CAPACITY)
         return GridNeighborhoodSearch(self.ndims, search_radius=r, n_points=self.n_points,
                                       cell_list=cl, update_strategy=self.update_strategy)
+
+    def _update_inner(self, x, y, points_moving, rebuild_lists):
+        """Updates the inner search and the lists.  A grid this search made for itself
+        covers the bounding box it was made for; when points have left it (the grid's
+        bounds error), it is remade over the current points and the update repeated, so
+        points may move anywhere, as with the reference's unbounded default."""
+        if self.neighborhood_search is None:
+            self.neighborhood_search = self._auto_grid(x, y)
+            self._auto = True
+        try:
+            self.neighborhood_search.update(x, y, points_moving=points_moving)
+            if rebuild_lists:
+                self._build_lists(x, y)
+        except (PointNeighborsError, BoundsError):
+            if not self._auto:
+                raise
+            self.neighborhood_search = self._auto_grid(x, y)
+            self.neighborhood_search.update(x, y, points_moving=points_moving)
+            if rebuild_lists:
+                self._build_lists(x, y)
 
     def initialize(self, x, y, *, eachindex_y=None):
         """src/nhs_precomputed.jl:130-147"""
@@ -156,10 +184,7 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
             raise PointNeighborsError("this neighborhood search does not support inactive points")
         if self._frozen:
             raise PointNeighborsError("a frozen neighborhood search cannot be initialized")
-        if self.neighborhood_search is None:
-            self.neighborhood_search = self._auto_grid(x, y)
-        self.neighborhood_search.initialize(x, y)
-        self._build_lists(x, y)
+        self._update_inner(x, y, (True, True), True)
         return self
 
     def update(self, x, y, *, points_moving=(True, True), eachindex_y=None):
@@ -168,11 +193,7 @@ class PrecomputedNeighborhoodSearch(metaclass=_GridMeta):
             raise PointNeighborsError("this neighborhood search does not support inactive points")
         if self._frozen:
             raise PointNeighborsError("a frozen neighborhood search cannot be updated")
-        if self.neighborhood_search is None:
-            self.neighborhood_search = self._auto_grid(x, y)
-        self.neighborhood_search.update(x, y, points_moving=points_moving)
-        if any(points_moving):
-            self._build_lists(x, y)
+        self._update_inner(x, y, points_moving, any(points_moving))
         return self
 
     def _build_lists(self, x, y):

@@ -227,6 +227,67 @@ def test_different_point_sets_and_subsets(orc):
     assert np.array_equal(gpu_counts(nhs, dy, dy), ocnt2)
 
 
+def test_repeated_selections(orc):
+    """A point selected twice is visited twice, as the reference's loops over `points` and
+    `eachindex_y` do: a repeated y is binned twice (each copy its own slot in the cell, so
+    counts and index sums see it twice); a repeated query point gets its result twice."""
+    r = 2.5
+    y = cloud((30, 32), 4)
+    nhs = make_nhs(y, r, y.min(axis=0) - r, y.max(axis=0) + r)
+    dy = dev(y)
+    n = len(y)
+    active = np.concatenate([np.arange(n), [5, 5, 700, 3]]).astype(np.int64)
+    pn.initialize(nhs, dy, dy, eachindex_y=torch.as_tensor(active, device="cuda"))
+    og = oracle_like(orc, nhs, np.float64)
+    assert og.build(y, active=active)[0] == orc.OK
+    _, _, ocnt, osum, _ = og.traverse(y, y)
+    assert np.array_equal(gpu_counts(nhs, dy, dy), ocnt)
+    assert np.array_equal(gpu_sums(nhs, dy, dy), osum)
+    start, idx = nhs.cell_table()
+    assert np.array_equal(np.sort(idx), np.sort(active))
+    for c in range(len(start) - 1):  # canonical order: ascending index inside every cell
+        assert (np.diff(idx[start[c]:start[c + 1]]) >= 0).all()
+    pn.initialize(nhs, dy, dy)
+    assert og.build(y)[0] == orc.OK
+    _, _, ocnt, _, _ = og.traverse(y, y)
+    c = gpu_counts(nhs, dy, dy, points=[3, 3, 17, 959, 17])
+    mask = np.zeros(n, dtype=bool)
+    mask[[3, 17, 959]] = True
+    assert np.array_equal(c[mask], ocnt[mask]) and not c[~mask].any()
+
+
+def test_coordinates_written_through_the_c_abi_are_reread(orc):
+    """y moved in place by a C-ABI kernel (the replayable displacement) after update!: the next
+    traversal reads the moved coordinates with the cells of the last update!, as the reference
+    reads current coordinates (src/nhs_grid.jl:543-545)."""
+    from pointneighbors import batch
+    r = 2.5
+    a = cloud((25, 25), 1)
+    nhs = make_nhs(a, r, a.min(axis=0) - 2 * r, a.max(axis=0) + 2 * r)
+    x = dev(a)
+    pn.initialize(nhs, x, x)
+    step = torch.zeros(1, dtype=torch.int64, device="cuda")
+    batch.displace_uniform(x, 0.3, 5, step)
+    b = x.t().cpu().numpy().copy()
+    assert not np.array_equal(a, b)
+    og = oracle_like(orc, nhs, np.float64)
+    assert og.build(a)[0] == orc.OK
+    _, _, ocnt, osum, _ = og.traverse(b, b)
+    assert np.array_equal(gpu_counts(nhs, x, x), ocnt)
+    assert np.array_equal(gpu_sums(nhs, x, x), osum)
+    # a write torch cannot see, declared with coordinates_changed()
+    raw = x.clone()
+    pn.update(nhs, x, x)
+    torch.cuda.synchronize()
+    pn.foreach_point_neighbor(pn.CountNeighbors(torch.zeros(len(a), dtype=torch.int32, device="cuda")), x, x, nhs)
+    x.data.copy_(dev(a))  # through .data: the version counter does not move
+    nhs.coordinates_changed()
+    og2 = oracle_like(orc, nhs, np.float64)
+    assert og2.build(raw.t().cpu().numpy())[0] == orc.OK
+    _, _, ocnt2, _, _ = og2.traverse(a, a)
+    assert np.array_equal(gpu_counts(nhs, x, x), ocnt2)
+
+
 def test_empty_neighborhood_returns_init():
     r = 2.5
     coords = cloud((10, 11), 1)
@@ -360,6 +421,29 @@ def test_config_wcsph_1m_neighbor_sets(orc):
 
 
 # ---- precomputed ------------------------------------------------------------------------------------
+
+def test_precomputed_own_grid_follows_moving_points(orc):
+    """Without a user grid the search makes one over the points' bounding box; update! after
+    the points moved far outside it, or packed more than 100 into a cell, still succeeds
+    (the reference's default inner search is unbounded) and the lists equal the oracle's."""
+    r = 2.5
+    a = cloud((20, 21), 3)
+    # lists of up to 1000 neighbours (the 150 packed points exceed the 2D default of 64, which
+    # the reference's lists would reject too)
+    p = pn.PrecomputedNeighborhoodSearch(2, search_radius=r, n_points=len(a), max_neighbors=1000)
+    x = dev(a)
+    pn.initialize(p, x, x)
+    moved = a * np.array([3.0, 0.5]) + np.array([500.0, -80.0])
+    moved[:150] = moved[0] + (synthetic.uniform(3, 300).reshape(150, 2) - 0.5) * 0.4  # 150 in one cell
+    xm = dev(moved)
+    pn.update(p, xm, xm)
+    nl = p.neighbor_lists
+    rp = nl.row_ptr.cpu().numpy().astype(np.int64)
+    cols = nl.col_idx.cpu().numpy()
+    orp, ocols = orc.trivial(moved, moved, r, lists=True)[1]  # ascending lists
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(cols, ocols)
+
 
 def test_precomputed_lists_and_freeze(orc, golden):
     r = 2.5

@@ -128,9 +128,14 @@ def test_sph_density_matches_full_grid(dt, tol):
 
 def test_stale_coordinates_follow_current_cells():
     """Points moved in place without update!: buckets come from the last update!, and a
-    candidate counts only if its current cell is the scanned cell (the reference's
-    check_collision on every bucket, as its parallel build flags them all).  Expected
-    counts from that loop restated in numpy (src/nhs_grid.jl:519-575)."""
+    candidate counts only if its current cell is the scanned cell (check_collision,
+    src/nhs_grid.jl:479-513), applied to every bucket.  The reference's parallel build never
+    records a bucket's cell (push_cell_atomic! compares a UInt128 with an NTuple,
+    src/cell_lists/spatial_hashing.jl:103-116), so it flags every bucket that holds a point of
+    a cell other than (0, ..., 0) and checks those; a bucket holding only points of cell
+    (0, ..., 0) stays unflagged and its stale points are accepted unchecked.  This build checks
+    that bucket too (a recorded deviation).  Expected counts from the checked loop restated in
+    numpy (src/nhs_grid.jl:519-575)."""
     r = 2.5
     a = cloud((25, 25), 1)
     b = a + (synthetic.uniform(9, a.size).reshape(a.shape) - 0.5) * 0.6

@@ -282,3 +282,167 @@ def test_tlsph_momentum_and_affine_penalty(orc, nd):
     pen = orc.tlsph_force(X, xa, lists, m, rho, Pr, h, F=FA, young_modulus=3.0, penalty_alpha=0.1)
     assert np.allclose(pen, base, rtol=0, atol=1e-12 * np.abs(base).max())
     assert not np.allclose(dv, orc.tlsph_force(X, x, lists, m, rho, Pr, h), rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("kernel", ["cubic_spline", "wendland_c2"])
+def test_threaded_rows_equal_list_sums(orc, kernel):
+    """The threaded grid-based density / pressure sums (used at 1M and 16.8M points) equal the
+    list-based sums on the same Float32 cloud, for all rows and for a scattered subset."""
+    sp = 0.01
+    pts = synthetic.jittered_lattice((17, 18, 19), sp, 0.1 * sp, dtype=np.float32, seed=8)
+    h = np.float32(1.2) * np.float32(sp)
+    r = np.float32(2) * h
+    lo, hi = pts.min(axis=0) - 3 * r, pts.max(axis=0) + 3 * r
+    pmin, _, ncells, _ = orc.fullgrid_geometry(lo, hi, r, np.float32, np.float32)
+    g = orc.Grid(3, pmin, ncells, float(r), coord_dtype=np.float32, radius_dtype=np.float32,
+                 cell_dtype=np.float32)
+    st, counts, _, _ = g.threaded_count_f32(pts, 4)
+    assert st == orc.OK
+    st, _, lists = g.precomputed(pts, pts, 100000)
+    assert st == orc.OK
+    assert np.array_equal(np.diff(lists[0]), counts)
+    n = len(pts)
+    m = synthetic.uniform(3, n) + 0.5
+    rho, rs = orc.sph_density(pts, lists, m, float(h), kernel)
+    P = synthetic.uniform(4, n) - 0.3
+    dv, ds = orc.sph_pressure_accel(pts, lists, m, P, float(h), kernel)
+    for rows in (np.arange(n), np.array([0, 5, n - 1, 777, 4000, 12])):
+        st, rho2, rs2, dv2, ds2 = g.threaded_sph_rows_f32(pts, rows, 3, m, float(h), kernel, P=P)
+        assert st == orc.OK
+        np.testing.assert_allclose(rho2, rho[rows], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(rs2, rs[rows], rtol=1e-13, atol=0)
+        assert (np.abs(dv2 - dv[rows]).max(axis=1) <= 1e-13 * ds[rows]).all()
+        np.testing.assert_allclose(ds2, ds[rows], rtol=1e-13, atol=0)
+
+
+# ---- n-body closure (benchmarks/n_body.jl:34-51) -------------------------------------------
+
+def two_point_lists():
+    return (np.array([0, 2, 4]), np.array([0, 1, 0, 1], dtype=np.int32))
+
+
+def test_nbody_two_bodies_and_conservation(orc):
+    """Two bodies attract along their axis with G m_other / d^2; the self pair and coincident
+    bodies contribute nothing; sum_i m_i dv_i = 0 over symmetric lists."""
+    G, d = 6.6743e-11, 0.37
+    x = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]])
+    m = np.array([2e10, 3e10])
+    dv, _ = orc.nbody(x, two_point_lists(), m, G)
+    assert dv[0] == pytest.approx([G * m[1] / d**2, 0, 0], rel=1e-14)
+    assert dv[1] == pytest.approx([-G * m[0] / d**2, 0, 0], rel=1e-14)
+    same = np.zeros((2, 3))
+    dv, _ = orc.nbody(same, two_point_lists(), m, G)
+    assert not dv.any()
+    pts = synthetic.jittered_lattice((9, 10, 8), 1.0, 0.3, dtype=np.float64, seed=5)
+    lists = neighbor_lists(orc, pts, 2.5)
+    m = 1e10 * (synthetic.uniform(5, len(pts)) + 1)
+    dv, scale = orc.nbody(pts, lists, m, G)
+    assert np.abs((m[:, None] * dv).sum(axis=0)).max() <= 1e-13 * (m * scale).sum()
+
+
+# ---- WCSPH interaction (TrixiParticles interact!, benchmark_wcsph) ------------------------------
+
+WC = dict(h=0.6, c=10.0, kernel="wendland_c2", alpha=0.02, beta=0.0, epsilon=0.01, delta=0.1)
+
+
+def wendland_deriv_fd(orc, d, h):
+    """W'(d) by central differences of the oracle's density of a two-point system (m = 1)."""
+    s = 1e-6
+    vals = []
+    for dd in (d + s, d - s):
+        x = np.array([[0.0, 0.0, 0.0], [dd, 0.0, 0.0]])
+        rho, _ = orc.sph_density(x, two_point_lists(), np.ones(2), h, "wendland_c2")
+        vals.append(rho[0])
+    return (vals[0] - vals[1]) / (2 * s)
+
+
+def test_wcsph_two_particle_terms(orc):
+    """Closed forms of each term for two particles on the x axis (a at 0, b at d > 0), with W'
+    from finite differences of the density sum: pressure repels, viscosity opposes an
+    approach, an approach raises the density, diffusion lowers the denser particle's density."""
+    h, c, d, m = WC["h"], WC["c"], 0.7, 0.3
+    x = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]])
+    L = two_point_lists()
+    dW = wendland_deriv_fd(orc, d, h)
+    assert dW < 0
+    # pressure only (at rest, one density)
+    v = np.array([[0, 0, 0, 1000.0], [0, 0, 0, 1000.0]])
+    p = np.array([40.0, 60.0])
+    dv, _ = orc.wcsph_interact(x, L, np.full(2, m), v, p, h, c, **{k: WC[k] for k in ("kernel", "alpha", "beta",
+                                                                                   "epsilon", "delta")})
+    # grad_a W = W'(d) / d * (x_a - x_b) = -W'(d) e_x; dv_a = -m (p_a + p_b) / rho^2 grad_a W
+    want = -m * 100.0 / 1000.0**2 * (-dW)
+    assert dv[0, 0] == pytest.approx(want, rel=1e-7) and dv[0, 0] < 0
+    assert dv[1, 0] == pytest.approx(-want, rel=1e-7)
+    assert dv[0, 3] == 0.0 and not dv[:, 1:3].any()
+    # approach at speed 2u: viscosity and continuity, no pressure
+    u = 0.5
+    v = np.array([[u, 0, 0, 1000.0], [-u, 0, 0, 1000.0]])
+    dv, _ = orc.wcsph_interact(x, L, np.full(2, m), v, np.zeros(2), h, c, **{k: WC[k] for k in (
+        "kernel", "alpha", "beta", "epsilon", "delta")})
+    vr = 2 * u * -d
+    mu = h * vr / (d**2 + WC["epsilon"] * h**2)
+    pi = -WC["alpha"] * c * mu / 1000.0
+    assert dv[0, 0] == pytest.approx(-m * pi * (-dW), rel=1e-7) and dv[0, 0] < 0
+    assert dv[0, 3] == pytest.approx(m * 2 * u * (-dW), rel=1e-7) and dv[0, 3] > 0
+    # receding pair: no viscosity, the density falls
+    dv2, _ = orc.wcsph_interact(x, L, np.full(2, m), -v * np.array([1, 1, 1, -1]), np.zeros(2), h, c,
+                                **{k: WC[k] for k in ("kernel", "alpha", "beta", "epsilon", "delta")})
+    assert dv2[0, 0] == 0.0 and dv2[0, 3] == pytest.approx(-dv[0, 3], rel=1e-12)
+    # diffusion at rest: rho_a > rho_b
+    v = np.array([[0, 0, 0, 1010.0], [0, 0, 0, 1000.0]])
+    dv, _ = orc.wcsph_interact(x, L, np.full(2, m), v, np.zeros(2), h, c, **{k: WC[k] for k in (
+        "kernel", "alpha", "beta", "epsilon", "delta")})
+    # psi . grad W = 2 (rho_a - rho_b) (x_ab . grad W) / d^2 = 2 * 10 * (d W'(d)) / d^2
+    want = WC["delta"] * h * c * 2 * 10.0 * dW / d * m / 1000.0
+    assert dv[0, 3] == pytest.approx(want, rel=1e-7) and dv[0, 3] < 0
+
+
+def test_wcsph_conservation_and_fields(orc):
+    """Random velocities, densities, pressures and masses on a jittered cloud: total momentum
+    sum_a m_a dv_a = 0 and, at rest, the diffused volume sum_a (m_a / rho_a) drho_a = 0; a
+    uniform velocity field changes nothing; with one density the continuity term is the rate of
+    change of the summation density along a linear velocity field, and with one pressure and
+    density the pressure term is -2p/rho^2 times its gradient (finite differences)."""
+    pts = synthetic.jittered_lattice((8, 9, 7), 0.3, 0.06, dtype=np.float64, seed=3)
+    n = len(pts)
+    h = 0.3
+    r = 2 * h
+    lists = neighbor_lists(orc, pts, r)
+    kw = {k: WC[k] for k in ("kernel", "alpha", "beta", "epsilon", "delta")}
+    m = 0.02 * (1 + synthetic.uniform(7, n))
+    rho = 1000 + 5 * synthetic.uniform(8, n)
+    vel = synthetic.uniform(9, 3 * n).reshape(n, 3) - 0.5
+    p = 10 + 90 * synthetic.uniform(10, n)
+    v = np.concatenate([vel, rho[:, None]], axis=1)
+    dv, scale = orc.wcsph_interact(pts, lists, m, v, p, h, 10.0, **kw)
+    assert np.abs((m[:, None] * dv[:, :3]).sum(axis=0)).max() <= 1e-13 * (m * scale[:, 0]).sum()
+    rest = np.concatenate([np.zeros((n, 3)), rho[:, None]], axis=1)
+    dr, sr = orc.wcsph_interact(pts, lists, m, rest, p, h, 10.0, **kw)
+    assert abs((m / rho * dr[:, 3]).sum()) <= 1e-13 * (m / rho * sr[:, 1]).sum()
+    shifted = rest.copy()
+    shifted[:, :3] = [0.3, -0.2, 0.7]
+    ds, _ = orc.wcsph_interact(pts, lists, m, shifted, p, h, 10.0, **kw)
+    np.testing.assert_allclose(ds, dr, rtol=0, atol=1e-12 * np.abs(dr).max())
+    # continuity along v = A x, one density and mass, no diffusion or viscosity contribution
+    A = np.array([[0.2, -0.1, 0.05], [0.3, 0.1, 0.0], [-0.2, 0.4, -0.3]])
+    lin = np.concatenate([pts @ A.T, np.full((n, 1), 1000.0)], axis=1)
+    mm = np.full(n, 0.02)
+    dl, _ = orc.wcsph_interact(pts, lists, mm, lin, np.zeros(n), h, 10.0, **dict(kw, alpha=0.0))
+    s = 1e-6
+    rp_, rm_ = (orc.sph_density(pts + sg * s * (pts @ A.T), lists, mm, h, "wendland_c2")[0] for sg in (1, -1))
+    np.testing.assert_allclose(dl[:, 3], (rp_ - rm_) / (2 * s), rtol=1e-6, atol=1e-6 * np.abs(dl[:, 3]).max())
+    # pressure: -2p/rho^2 grad_a rho_a (the gradient of a point's summation density w.r.t. itself)
+    one = np.concatenate([np.zeros((n, 3)), np.full((n, 1), 1000.0)], axis=1)
+    dp, _ = orc.wcsph_interact(pts, lists, mm, one, np.full(n, 50.0), h, 10.0, **kw)
+    for a in (0, 17, n // 2, n - 5):
+        g = []
+        for dim in range(3):
+            vals = []
+            for sg in (1, -1):
+                xx = pts.copy()
+                xx[a, dim] += sg * s
+                vals.append(orc.sph_density(xx, lists, mm, h, "wendland_c2")[0][a])
+            g.append((vals[0] - vals[1]) / (2 * s))
+        np.testing.assert_allclose(dp[a, :3], -2 * 50.0 / 1000.0**2 * np.array(g), rtol=1e-6,
+                                   atol=1e-9 * np.abs(dp[:, :3]).max())

@@ -216,6 +216,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(pts, lo, hi, r, h, args.workload, args.cpu_seconds)
+    copy_gbs = streaming_copy_gbs(dev)
     if args.copy_probe:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -250,12 +251,32 @@ def main():
         "wall_s": wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                     "algorithmic_bytes_per_launch": algo},
+                     "algorithmic_bytes_per_launch": algo, "measured_copy_gbs": copy_gbs,
+                     "frac_of_measured_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "tuning": tuned,
         "autotune_table": tune_table,
     }
     print(json.dumps(line))
+
+
+def streaming_copy_gbs(dev, mib=1024, reps=10):
+    """The box's streaming bandwidth: a device-to-device copy of `mib` MiB (read + write bytes
+    over the best of `reps` timed copies), reported beside the nominal 8 TB/s."""
+    import torch
+    src = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    best = float("inf")
+    for _ in range(reps + 2):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    del src, dst
+    return 2 * mib * 1024 * 1024 / (best * 1e-3) / 1e9
 
 
 BATCH_CLOUDS, BATCH_MIN, BATCH_MAX, BATCH_R = 512, 1000, 65536, 2.4

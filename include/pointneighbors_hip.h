@@ -386,6 +386,23 @@ int pn_list_nbody_accel(const int32_t* row_ptr, const int32_t* col_idx, int64_t 
                         int64_t sel_first, int64_t sel_count, const int32_t* sel_list, void* dv,
                         int accumulate, void* stream);
 
+/* Allocation-free initialize! + neighbour count.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Equals pn_grid_create + pn_grid_update +
+ * pn_count_neighbors(PN_QUERIES_GRID, accumulate = 0) on the same arguments (geometry as
+ * pn_grid_create, y: D x N, counts: N int32), enqueued on `stream`, then synchronizes and
+ * returns the reference's errors (PN_ERR_OUT_OF_BOUNDS, PN_ERR_CELL_FULL).  Every buffer
+ * beyond y and counts lives in a static device array of pn_noalloc_scratch_bytes() bytes per
+ * GPU (64 MiB: about 28 bytes per Float32 3D point plus 12 bytes per cell); larger problems
+ * return PN_ERR_UNSUPPORTED before launching anything.  No hipMalloc or hipFree is called.
+ * Calls that overlap on one GPU (several threads, or concurrent streams) share that array
+ * and race. */
+int pn_count_neighbors_noalloc(int ndims, int coord_dtype, int radius_dtype, int cell_dtype,
+                               const double* min_corner, const int64_t* n_cells, double search_radius,
+                               int32_t max_points_per_cell, const void* y, int64_t n, int32_t* counts,
+                               void* stream);
+int64_t pn_noalloc_scratch_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 # translation units, compiled in parallel (each instantiates a share of the kernels)
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("pn_api.hip", "pn_api_trav.hip", "pn_api_sph.hip", "pn_api_vjp.hip", "pn_api_ext.hip")]
+           ("pn_api.hip", "pn_api_trav.hip", "pn_api_sph.hip", "pn_api_vjp.hip", "pn_api_ext.hip",
+            "pn_api_noalloc.hip")]
 OUT = os.path.join(HERE, "libpointneighbors_hip.so")
 
 HIPCC_FLAGS = [

@@ -106,6 +106,9 @@ SIGNATURES = {
     "pn_wcsph_interact": (_c_int, [_vp, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _vp, _vp, _vp,
                                    _vp, _c_int]),
     "pn_mapreduce_neighbors": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+    "pn_count_neighbors_noalloc": (_c_int, [_c_int, _c_int, _c_int, _c_int, _p_dbl, _p_i64, _c_dbl, _c_i32, _vp,
+                                            _c_i64, _vp, _vp]),
+    "pn_noalloc_scratch_bytes": (_c_i64, []),
     "pn_list_sph_density_periodic": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                               _vp, _vp, _vp, _c_i64, _c_i64, _vp, _vp, _c_int, _c_dbl, _p_dbl,
                                               _vp]),

@@ -131,7 +131,12 @@ def load(path: str | None = None):
             f"{p} is missing: build it with `python pointneighbors.jl_amd/build.py` "
             "(the MI355X path has no CPU fallback)")
     lib = ctypes.CDLL(p)
+    # another build named by POINTNEIGHBORS_HIP_LIB may predate some entry points; the in-tree
+    # build must export them all
+    other = p != os.path.join(os.path.dirname(_HERE), "libpointneighbors_hip.so")
     for name, (res, args) in SIGNATURES.items():
+        if other and not hasattr(lib, name):
+            continue
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args

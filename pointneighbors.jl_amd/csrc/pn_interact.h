@@ -45,8 +45,10 @@ struct NBodyF : FunctorBase {
     }
     template <class T, int DD>
     __device__ void pair(Acc& a, const QState&, const T (&pd)[DD], T d2, int, const T* fj, int*) const {
-        const T dist = psqrt(d2);
-        const T c = dist < sqrt_eps<T>() ? (T)0 : -G * fj[0] / (dist * dist * dist);
+        // 1/d^3 from the reciprocal square root (no division); d < sqrt(eps) adds nothing
+        const T inv_d = prsqrt0(d2);
+        const T dist = d2 * inv_d;
+        const T c = dist < sqrt_eps<T>() ? (T)0 : -G * fj[0] * (inv_d * inv_d * inv_d);
 #pragma unroll
         for (int d = 0; d < D; ++d) a.v[d] = pfma(c, pd[d], a.v[d]);
     }
@@ -81,7 +83,7 @@ struct WCSPHInteractF : FunctorBase {
     static constexpr int kFields = 3 + D;
     struct QState {
         TR v[D];
-        TR rho, p;
+        TR rho, p, inv_rho;
     };
     SmoothingKernel<TR, D> kern;
     WCSPHParams<TR> prm;
@@ -95,6 +97,7 @@ struct WCSPHInteractF : FunctorBase {
         for (int d = 0; d < D; ++d) q.v[d] = vmat[(D + 1) * i + d];
         q.rho = vmat[(D + 1) * i + D];
         q.p = pressure[i];
+        q.inv_rho = prcp(q.rho);
         return q;
     }
     __device__ Acc init() const {
@@ -113,14 +116,16 @@ struct WCSPHInteractF : FunctorBase {
         T vr = (T)0;
 #pragma unroll
         for (int d = 0; d < D; ++d) vr = pfma(q.v[d] - fj[3 + d], pd[d], vr);
-        const T inv_rhob = (T)1 / rhob;
-        // pressure and viscosity (the viscosity only for approaching pairs)
-        T coef = -mb * (q.p + pb) / q.rho * inv_rhob;
-        if (vr < (T)0) {
-            const T mu = prm.h * vr / (d2 + prm.eps_h2);
-            const T rho_mean = (T)0.5 * (q.rho + rhob);
-            const T pi = pfma(prm.beta * mu, mu, -prm.alpha * prm.c * mu) / rho_mean;
-            coef = pfma(-mb, pi, coef);
+        const T inv_rhob = prcp(rhob);
+        // pressure and viscosity (the viscosity only for approaching pairs, selected per lane);
+        // 1 / rho_mean = 2 / (rho_a + rho_b)
+        T coef = -mb * (q.p + pb) * q.inv_rho * inv_rhob;
+        // evaluated only when a lane of the wavefront has an approaching pair (a fluid at rest
+        // skips it; the Float64 reciprocals are divisions)
+        if (__builtin_amdgcn_ballot_w64(vr < (T)0)) {
+            const T mu = prm.h * vr * prcp(d2 + prm.eps_h2);
+            const T pi = pfma(prm.beta, mu, -prm.alpha * prm.c) * mu * ((T)2 * prcp(q.rho + rhob));
+            coef = vr < (T)0 ? pfma(-mb, pi, coef) : coef;
         }
         const T s = coef * gw;
 #pragma unroll
@@ -241,8 +246,9 @@ __global__ __launch_bounds__(256) void k_list_nbody(const int32_t* __restrict__ 
                 const TR p2 = pd[d] * pd[d];
                 d2 = d2 + p2;
             }
-            const TR dist = psqrt(d2);
-            const TR c = (j[u] < 0 || dist < sqrt_eps<TR>()) ? (TR)0 : -G * mv[u] / (dist * dist * dist);
+            const TR inv_d = prsqrt0(d2);
+            const TR dist = d2 * inv_d;
+            const TR c = (j[u] < 0 || dist < sqrt_eps<TR>()) ? (TR)0 : -G * mv[u] * (inv_d * inv_d * inv_d);
 #pragma unroll
             for (int d = 0; d < D; ++d) s[d] = pfma(c, pd[d], s[d]);
         }

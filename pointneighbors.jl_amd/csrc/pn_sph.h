@@ -50,6 +50,9 @@ template <>
 __device__ __forceinline__ double sqrt_eps<double>() {
     return 1.4901161193847656e-8;  // sqrt(2^-52) = 2^-26
 }
+// 1/x: the hardware reciprocal (about 1 ulp) in Float32, IEEE division in Float64
+__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float pfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double pfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 

@@ -173,6 +173,7 @@ int pn_grid_refresh(pn_grid_t grid, const void* y, int64_t n_y) {
     if (!s.valid) return fail(PN_ERR_NOT_INITIALIZED, "the neighborhood search is not initialized");
     if (n_y != s.n_src) return fail(PN_ERR_ARGUMENT, "coordinate matrix size differs from the last update!");
     ++grid->gen;  // coordinates changed: recorded pair lists are stale
+    s.fresh = false;  // and so are a hash grid's per-bucket cells
     if (s.n_sel == 0) return PN_OK;
     return dispatch_build(*grid, [&](auto dc, auto tcv, auto) -> int {
         constexpr int D = decltype(dc)::value;

@@ -89,6 +89,10 @@ struct SetView {
     const void* coords;         // D planes of cap entries, cell-sorted
     int64_t cap;
     const DevStatus* st;
+    // SpatialHashingCellList only (else NULL), as of the last binning: per bucket the cell of
+    // its first point (3 int32) and whether its points lie in more than one cell
+    const int32_t* bucket_cell;
+    const int32_t* bucket_mixed;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that each XCD
@@ -463,6 +467,39 @@ __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
     sorted_idx[pos] = p;
 #pragma unroll
     for (int d = 0; d < D; ++d) coords[(int64_t)d * cap + pos] = y[(int64_t)D * p + d];
+}
+
+// SpatialHashingCellList: per non-empty bucket, the cell of its first point and whether any
+// of its points lies in another cell (a hash collision), from the sorted coordinates of the
+// binning just done (the reference's collision flags, src/cell_lists/spatial_hashing.jl:
+// 84-116, computed once instead of per push).  One thread per non-empty bucket.
+template <int D, class TC, class TCELL>
+__global__ __launch_bounds__(256) void k_bucket_info(Geom g, SetView S, int32_t* __restrict__ cell,
+                                                     int32_t* __restrict__ mixed) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.st->n_nonempty) return;
+    const int b = S.nonempty[t];
+    const int s = S.start[b], e = S.start[b + 1];
+    const TC* __restrict__ x = (const TC*)S.coords;
+    auto cell_of = [&](int k, int64_t (&c)[3]) {
+        TC p[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[d] = x[(int64_t)d * S.cap + k];
+        bool ok;
+        hash_cell<D, TC, TCELL>(g, p, c, ok);
+    };
+    int64_t c0[3];
+    cell_of(s, c0);
+    int mx = 0;
+    for (int k = s + 1; k < e && !mx; ++k) {
+        int64_t c[3];
+        cell_of(k, c);
+#pragma unroll
+        for (int d = 0; d < D; ++d) mx |= c[d] != c0[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cell[3 * (int64_t)b + d] = (int32_t)c0[d];
+    mixed[b] = mx;
 }
 
 // Re-gathers the sorted coordinate planes from y (update-free coordinate refresh).

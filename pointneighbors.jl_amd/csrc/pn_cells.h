@@ -90,6 +90,15 @@ struct CellGroups {
 template <int N>
 using IntC = std::integral_constant<int, N>;
 
+// 64 / n for n = 0 .. 64 (0 -> 64): the lanes per query of a flush, read with a scalar load.
+struct LanesTable {
+    uint8_t v[WAVE + 1];
+    constexpr LanesTable() : v{} {
+        for (int n = 0; n <= WAVE; ++n) v[n] = (uint8_t)(WAVE / (n > 0 ? n : 1));
+    }
+};
+static __constant__ constexpr LanesTable kLanesTable{};
+
 // Calls fn(IntC<n>{}) for a runtime n in 1..G, so loops over n groups unroll without guards.
 template <int G, class Fn>
 __device__ __forceinline__ void with_groups(int n, Fn&& fn) {
@@ -104,11 +113,19 @@ __device__ __forceinline__ void with_groups(int n, Fn&& fn) {
 
 // PCM: pair cache mode of this instance (0 off, 1 record, 2 replay); only functors with
 // kCache use 1 and 2.
-template <int D, class TC, class TR, class F, int G, bool PER, int PCM>
+// HASH: the grid is keyed by a SpatialHashingCellList (a work item is a non-empty query
+// bucket, the pieces are the 3^D neighbour cells' buckets) and binned from the current
+// coordinates, so the per-bucket cell and collision flags of the binning hold.  A query
+// bucket that mixes cells is run once per cell, over that cell's queries.
+template <int D, class TC, class TR, class F, int G, bool PER, int PCM, bool HASH = false>
 __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, TR r2, F f, int nblocks,
                                                 PairCache pc) {
+    static_assert(!HASH || PCM == 0, "no pair cache over a hash grid");
     constexpr int R = Rows<D>::R;
-    constexpr int RP = PER ? 2 * R : R;  // stencil row pieces
+    // stencil pieces: rows (two per row with a periodic box), or 3^D buckets
+    constexpr int RP = HASH ? (D == 1 ? 3 : (D == 2 ? 9 : 27)) : (PER ? 2 * R : R);
+    // the cell arithmetic of a hash grid (as the binning's)
+    using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
     constexpr int NF = F::kFields;
     constexpr int CH = G * WAVE;
     constexpr bool kLight = F::kLight;
@@ -245,7 +262,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         // L lanes per query (the largest power of two with L * nf <= 64): lane l takes query
         // q = l / L and its pairs qsp[q] + (l % L) + L t, so no lane changes query inside the
         // loop; the L partial sums of a query are then combined by a butterfly of fixed order.
-        const int L = WAVE / max(nf, 1);
+        const int L = kLanesTable.v[nf];
         // lane / L exactly: (lane + 1/2) / L is at least 1/(2L) away from an integer
         const int q = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)L)), sub = lane - q * L;
         Acc a = f.init();
@@ -314,6 +331,100 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
         cell_ok = true;
         const int qc = Q.nonempty[w];
         const int qs = Q.start[qc], qe = Q.start[qc + 1];
+        // The piece table in scalar registers: element m of piece u sits at sorted position
+        // m + rdel[u]; piece u holds elements [pre[u], pre[u + 1]).
+        // A hash grid (which needs each element's piece for its cell check) and more than 9
+        // pieces (a periodic box: 18) keep the table in lanes, lane u < RP holding pre[u] and
+        // rdel[u]: scalar registers would not hold 18 or 27 pieces, and the select chains over
+        // them became scratch-memory table loads.
+        constexpr bool kLaneTable = HASH || RP > 9;
+        int rdel[kLaneTable ? 1 : RP], pre[kLaneTable ? 1 : RP + 1];
+        int pre_l = 0, rdel_l = 0;
+        int M = 0;
+        // Lane u < RP holds piece u's first sorted position (rb) and length (rlen).
+        auto set_pieces = [&](int rb, int rlen) {
+            int pre_v = rlen;  // rlen of lanes <= u, summed
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const int t = __shfl_up(pre_v, o, WAVE);
+                if (lane >= o) pre_v += t;
+            }
+            M = __builtin_amdgcn_readlane(pre_v, RP - 1);
+            if constexpr (kLaneTable) {
+                pre_l = pre_v - rlen;
+                rdel_l = rb - pre_l;
+            } else {
+                pre[0] = 0;
+#pragma unroll
+                for (int t = 0; t < RP; ++t) {
+                    rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
+                    pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
+                }
+            }
+        };
+        // Lane table: piece tp of element m of the group of 64 elements from mlo, and its offset
+        // del (sorted position m + del).  The group spans pieces ulo..uhi (a few: found with two
+        // ballots); only those are scanned, with the table read from lanes.  Called with every
+        // lane of the wave active.
+        auto lane_piece = [&](int mlo, int m, int& del, int& tp) {
+            const bool inr = lane < RP;
+            const int ulo = __builtin_popcountll(__builtin_amdgcn_ballot_w64(inr && pre_l <= mlo)) - 1;
+            const int uhi = __builtin_popcountll(__builtin_amdgcn_ballot_w64(inr && pre_l <= mlo + WAVE - 1)) - 1;
+            del = __builtin_amdgcn_readlane(rdel_l, ulo);
+            tp = ulo;
+            for (int t = ulo + 1; t <= uhi; ++t) {
+                const int p = __builtin_amdgcn_readlane(pre_l, t);
+                const int r = __builtin_amdgcn_readlane(rdel_l, t);
+                del = m >= p ? r : del;
+                tp = m >= p ? t : tp;
+            }
+        };
+        // HASH: the cell whose 3^D neighbour cells the pieces are, and the pieces whose bucket
+        // holds points of several cells (their elements are checked one by one).
+        int64_t hc[3] = {0, 0, 0};
+        unsigned long long hchk = 0;
+        bool qmixed = false;
+        // Pieces: the buckets of the 3^D cells around hc in CartesianIndices order (first
+        // dimension fastest), as hash_query visits them.  A bucket whose points all lie in
+        // one other cell (a collision) contributes nothing; a bucket that mixes cells is
+        // scanned with a per-element cell check.
+        auto hash_pieces = [&]() {
+            int rb = 0, rlen = 0;
+            bool chk = false;
+            if (lane < RP) {
+                int64_t nc[3] = {hc[0], hc[1], hc[2]};
+                int t = lane;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    nc[d] = hc[d] + (t % 3) - 1;
+                    if constexpr (PER) nc[d] = hash_wrap(nc[d], g.per[d]);
+                    t /= 3;
+                }
+                const int b = spatial_hash<D>(nc, g.list_size);
+                rb = Gs.start[b];
+                rlen = Gs.start[b + 1] - rb;
+                if (rlen > 0) {
+                    if (Gs.bucket_mixed[b]) {
+                        chk = true;
+                    } else {
+                        bool same = true;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) same = same && (int64_t)Gs.bucket_cell[3 * (int64_t)b + d] == nc[d];
+                        if (!same) rlen = 0;
+                    }
+                }
+            }
+            hchk = __builtin_amdgcn_ballot_w64(chk);
+            set_pieces(rb, rlen);
+        };
+        if constexpr (HASH) {
+            qmixed = Q.bucket_mixed[qc] != 0;
+            if (!qmixed) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) hc[d] = Q.bucket_cell[3 * (int64_t)qc + d];
+                hash_pieces();
+            }
+        } else {
         int n0, n1;
         grid_dims_xy(g, qc, gn0, gn1, n0, n1);
         // Stencil rows, each one or two pieces of the sorted order: piece u covers
@@ -353,25 +464,32 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 }
             }
         }
-        // Piece table in lanes: lane u < RP holds piece u's first sorted position (rb), its
-        // length (rlen) and its first element index pre_v = rlen of lanes < u, summed.
-        int pre_v = rlen;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const int t = __shfl_up(pre_v, o, WAVE);
-            if (lane >= o) pre_v += t;
+        set_pieces(rb, rlen);
         }
-        const int M = __builtin_amdgcn_readlane(pre_v, RP - 1);
-        pre_v -= rlen;
-        // The piece table in scalar registers: element m of piece u sits at sorted position
-        // m + rdel[u].
-        int rdel[RP], pre[RP + 1];
-        pre[0] = 0;
+
+        // HASH, pieces of a bucket that mixes cells: an element whose own cell (from its
+        // coordinates x) is not its piece's cell becomes +Inf, never accepted (hash_query's
+        // check, once per staged element).
+        auto hash_check = [&](int tp, TC* x) {
+            if ((hchk >> tp) & 1) {
+                int64_t cj[3];
+                bool okj;
+                hash_cell_p<D, TC, TCELL, PER>(g, x, cj, okj);
+                bool same = true;
+                int t = tp;
 #pragma unroll
-        for (int t = 0; t < RP; ++t) {
-            rdel[t] = __builtin_amdgcn_readlane(rb, t) - pre[t];
-            pre[t + 1] = pre[t] + __builtin_amdgcn_readlane(rlen, t);
-        }
+                for (int d = 0; d < D; ++d) {
+                    int64_t want = hc[d] + (t % 3) - 1;
+                    if constexpr (PER) want = hash_wrap(want, g.per[d]);
+                    same = same && cj[d] == want;
+                    t /= 3;
+                }
+                if (!same) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) x[d] = (TC)INFINITY;
+                }
+            }
+        };
 
         // Counting: loads chunk m0 (mc elements) into registers; group gi holds elements
         // m0 + 64 gi + lane, the piece of element m is the last u with pre[u] <= m.  Lanes
@@ -383,13 +501,21 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 if (gi < ng) {
                     const int ml = gi * WAVE + lane;
                     const int m = m0 + ml;
-                    int del = rdel[0];
+                    int del, tp = 0;
+                    if constexpr (kLaneTable) {
+                        lane_piece(m0 + gi * WAVE, m, del, tp);
+                    } else {
+                        del = rdel[0];
 #pragma unroll
-                    for (int t = 1; t < RP; ++t) del = m >= pre[t] ? rdel[t] : del;
+                        for (int t = 1; t < RP; ++t) del = m >= pre[t] ? rdel[t] : del;
+                    }
                     if (ml < mc) {
                         const unsigned src = (unsigned)(m + del);
 #pragma unroll
                         for (int d = 0; d < D; ++d) cx[gi][d] = gc[(int64_t)d * Gs.cap + src];
+                        if constexpr (HASH) {
+                            if (hchk) hash_check(tp, cx[gi]);
+                        }
                     } else {
 #pragma unroll
                         for (int d = 0; d < D; ++d) cx[gi][d] = (TC)INFINITY;
@@ -415,15 +541,27 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 jv[gi] = 0;
 #pragma unroll
                 for (int fi = 0; fi < (NF > 0 ? NF : 1); ++fi) fv[gi][fi] = (TR)0;
+                const int m = m0 + ml;
+                int hdel = 0, htp = 0;
+                if constexpr (kLaneTable) {
+                    if (gi < ng) lane_piece(m0 + gi * WAVE, m, hdel, htp);
+                }
                 if (gi < ng && ml < mc) {
-                    const int m = m0 + ml;
-                    // a sum of per-piece steps (a select chain here becomes a scratch table load)
-                    int del = rdel[0];
+                    int del;
+                    if constexpr (kLaneTable) {
+                        del = hdel;
+                    } else {
+                        // a sum of per-piece steps (a select chain here becomes a scratch table load)
+                        del = rdel[0];
 #pragma unroll
-                    for (int t = 1; t < RP; ++t) del += m >= pre[t] ? rdel[t] - rdel[t - 1] : 0;
+                        for (int t = 1; t < RP; ++t) del += m >= pre[t] ? rdel[t] - rdel[t - 1] : 0;
+                    }
                     const unsigned src = (unsigned)(m + del);
 #pragma unroll
                     for (int d = 0; d < D; ++d) pk[gi].v[d] = gc[(int64_t)d * Gs.cap + src];
+                    if constexpr (HASH) {
+                        if (hchk) hash_check(htp, pk[gi].v);
+                    }
                     if constexpr (F::kIndex) jv[gi] = Gs.sorted_idx[src];
                     if constexpr (NF > 0) {
 #pragma unroll
@@ -524,6 +662,32 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             int64_t orig_i;
             begin_batch(q0, nq, xq, orig_i);
             int mycount = 0;
+            // HASH, a bucket of several query cells: one pass per cell over the queries of
+            // the batch in that cell (qsel), the pieces rebuilt for it; else one pass.
+            unsigned long long qpend = 0, qsel = ~0ull;
+            int64_t qcl[3] = {0, 0, 0};
+            if constexpr (HASH) {
+                if (qmixed) {
+                    bool ok;
+                    hash_cell_p<D, TC, TCELL, PER>(g, xq, qcl, ok);
+                    qpend = __builtin_amdgcn_ballot_w64(qact);
+                }
+            }
+            do {
+            if constexpr (HASH) {
+                if (qmixed) {
+                    const int ld = __builtin_ctzll(qpend);
+                    bool same = qact;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        hc[d] = lane_bcast(qcl[d], ld);
+                        same = same && qcl[d] == hc[d];
+                    }
+                    qsel = __builtin_amdgcn_ballot_w64(same);
+                    qpend &= ~qsel;
+                    hash_pieces();
+                }
+            }
 
             for (int m0 = 0, chunk = 0; m0 < M; m0 += CH, ++chunk) {
                 const int mc = min(CH, M - m0);
@@ -554,6 +718,9 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     with_groups<G>(ng, [&](auto ngc) {
                         constexpr int NG = decltype(ngc)::value;
                         for (int i = 0; i < nq; ++i) {
+                            if constexpr (HASH) {
+                                if (!((qsel >> i) & 1)) continue;
+                            }
                             TC xs[D];
 #pragma unroll
                             for (int d = 0; d < D; ++d) xs[d] = s_qc[i].v[d];
@@ -568,16 +735,26 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     });
                 } else {
                     int P = 0, i0 = 0;
+                    int qfirst = 0;  // lane t: first list entry of query i0 + t
+                    // flushes queries i0 .. i1-1 (list starts to LDS first)
+                    auto flush_to = [&](int i1) {
+                        const int nf = i1 - i0;
+                        if (lane < nf) s_qstart[lane] = (uint16_t)qfirst;
+                        if (lane == 0) s_qstart[nf] = (uint16_t)P;
+                        flush(i0, i1, P, b, chunk, s_qstart, s_pairs);
+                    };
                     with_groups<G>(ng, [&](auto ngc) {
                         constexpr int NG = decltype(ngc)::value;
                         for (int i = 0; i < nq; ++i) {
                             if (P + NG * WAVE > PAIR_CAP) {
-                                if (lane == 0) s_qstart[i - i0] = (uint16_t)P;
-                                flush(i0, i, P, b, chunk, s_qstart, s_pairs);
+                                flush_to(i);
                                 i0 = i;
                                 P = 0;
                             }
-                            if (lane == 0) s_qstart[i - i0] = (uint16_t)P;
+                            qfirst = write_lane(qfirst, P, i - i0);
+                            if constexpr (HASH) {
+                                if (!((qsel >> i) & 1)) continue;  // no pairs in this pass
+                            }
                             TC xs[D];
 #pragma unroll
                             for (int d = 0; d < D; ++d) xs[d] = s_qc[i].v[d];
@@ -588,18 +765,19 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                                 const bool acc = test(xs, gi);
                                 const unsigned long long mask = __builtin_amdgcn_ballot_w64(acc);
                                 if (acc) {
-                                    const int below = __builtin_amdgcn_mbcnt_hi(
-                                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                                    s_pairs[P + below] = (PairT)(gi * WAVE + lane);
+                                    // P + (accepted lanes below this one)
+                                    const int slot = __builtin_amdgcn_mbcnt_hi(
+                                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)P));
+                                    s_pairs[slot] = (PairT)(gi * WAVE + lane);
                                 }
                                 P += __builtin_popcountll(mask);
                             }
                         }
                     });
-                    if (lane == 0) s_qstart[nq - i0] = (uint16_t)P;
-                    flush(i0, nq, P, b, chunk, s_qstart, s_pairs);
+                    flush_to(nq);
                 }
             }
+            } while (qpend);
             if constexpr (kLight) {
                 if (qact) f.write(orig_i, QS{}, mycount);
             } else {

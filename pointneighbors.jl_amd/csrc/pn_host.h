@@ -96,6 +96,7 @@ struct Scanner {
 // One binned point set: the grid's y or a query set x.
 struct BinSet {
     Buf counts, start, nonempty, keys, slots, perm_tmp, sorted_idx, coords, status;
+    Buf bucket_cell, bucket_mixed;  // SpatialHashingCellList: per-bucket cell and collision flag
     pn::DevStatus* host_status = nullptr;  // pinned
     Scanner scan;
     int64_t cap = 0;
@@ -103,6 +104,7 @@ struct BinSet {
     int64_t n_src = 0;        // columns of the source matrix
     const void* src = nullptr;
     bool valid = false;       // last binning succeeded
+    bool fresh = false;       // binned from the current coordinates (no refresh since)
     bool stats_known = false;
     pn::DevStatus stats{};
 
@@ -116,6 +118,8 @@ struct BinSet {
         sorted_idx.release();
         coords.release();
         status.release();
+        bucket_cell.release();
+        bucket_mixed.release();
         scan.block_sums.release();
         if (host_status) (void)hipHostFree(host_status);
     }
@@ -128,6 +132,8 @@ struct BinSet {
         v.coords = coords.p;
         v.cap = cap;
         v.st = status.as<pn::DevStatus>();
+        v.bucket_cell = bucket_cell.as<int32_t>();
+        v.bucket_mixed = bucket_mixed.as<int32_t>();
         return v;
     }
 };
@@ -225,6 +231,10 @@ int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
     if ((rc = s.start.ensure((size_t)(C + 1) * 4, g.stream))) return rc;
     if ((rc = s.nonempty.ensure((size_t)C * 4, g.stream))) return rc;
     if ((rc = s.status.ensure(sizeof(pn::DevStatus), g.stream))) return rc;
+    if (g.geom.hash) {
+        if ((rc = s.bucket_cell.ensure((size_t)C * 12, g.stream))) return rc;
+        if ((rc = s.bucket_mixed.ensure((size_t)C * 4, g.stream))) return rc;
+    }
     if (!s.host_status) PN_HIP(hipHostMalloc((void**)&s.host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
     if (n_sel > s.cap) {
         int64_t cap = std::max<int64_t>(n_sel, 64);
@@ -286,6 +296,14 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
                                s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
                                s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order);
             PN_CHECK_LAUNCH();
+            if (g.geom.hash) {
+                // at most min(buckets, points) non-empty buckets; the kernel reads the count
+                const int64_t nb = std::min<int64_t>(C, sel_count);
+                hipLaunchKernelGGL((pn::k_bucket_info<D, TC, TCELL>), dim3(blocks_for(nb, 256)), dim3(256), 0,
+                                   g.stream, g.geom, s.view(), s.bucket_cell.as<int32_t>(),
+                                   s.bucket_mixed.as<int32_t>());
+                PN_CHECK_LAUNCH();
+            }
         }
         return PN_OK;
     });
@@ -294,6 +312,7 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
     s.n_src = n_src;
     s.src = src;
     s.valid = true;  // provisional until checked
+    s.fresh = true;
     s.stats_known = false;
     return PN_OK;
 }
@@ -422,7 +441,10 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
-    if (g.geom.hash) {  // a SpatialHashingCellList: the per-query kernel over buckets
+    if (g.geom.hash && (g.algorithm == 1 || !(Q->fresh && g.grid.fresh))) {
+        // a SpatialHashingCellList whose coordinates moved since the binning (the per-bucket
+        // cells of the binning no longer hold: every candidate's cell is recomputed), or the
+        // per-query algorithm asked for
         using TCELL = std::conditional_t<(sizeof(TC) == 8 || sizeof(TR) == 8), double, float>;
         const bool large_h = g.chunk < 0 ? !F::kQueue : g.chunk == 1;
         auto launch_h = [&](auto chunk, auto perc) {
@@ -439,6 +461,25 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
         } else {
             if (large_h) launch_h(LargeC{}, std::false_type{});
             else launch_h(SmallC{}, std::false_type{});
+        }
+        PN_CHECK_LAUNCH();
+        return PN_OK;
+    }
+    if (g.geom.hash) {  // a SpatialHashingCellList binned from the current coordinates
+        auto launch_hc = [&](auto gc, auto perc) {
+            constexpr int GG = decltype(gc)::value;
+            constexpr bool PER = decltype(perc)::value;
+            hipLaunchKernelGGL((pn::k_cells<D, TC, TR, F, GG, PER, 0, true>), dim3(nblocks), dim3(pn::WAVE), 0,
+                               g.stream, g.geom, Q->view(), g.grid.view(), r2, f, nblocks, pn::PairCache{});
+        };
+        using Large = pn::IntC<pn::CellGroups<TC, F>::large>;
+        using Small = pn::IntC<pn::CellGroups<TC, F>::small>;
+        if (g.geom.periodic) {
+            if (g.chunk != 0) launch_hc(Large{}, std::true_type{});
+            else launch_hc(Small{}, std::true_type{});
+        } else {
+            if (g.chunk != 0) launch_hc(Large{}, std::false_type{});
+            else launch_hc(Small{}, std::false_type{});
         }
         PN_CHECK_LAUNCH();
         return PN_OK;

@@ -92,6 +92,7 @@ int pn_grid_create_spatial_hash(int ndims, int coord_dtype, int radius_dtype, do
     g->geom.periodic = 0;
     g->geom.hash = 1;
     g->geom.list_size = (int)list_size;
+    g->geom.list_magic = ~0ull / (unsigned long long)list_size;
     *out = g;
     return PN_OK;
 }

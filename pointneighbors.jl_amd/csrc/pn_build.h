@@ -43,6 +43,7 @@ struct Geom {
     int n_clouds;
     int hash;       // SpatialHashingCellList: keys are hash buckets of unbounded cells
     int list_size;  // its number of buckets (== ncells)
+    unsigned long long list_magic;  // floor((2^64 - 1) / list_size), for the bucket reduction
 };
 
 // Cloud of a linear cell index in a batch (binary search over cell_offsets).
@@ -78,7 +79,7 @@ struct DevStatus {
     int max_count;                 // largest cell population
     int n_nonempty;                // number of non-empty cells
     int n_binned;                  // number of binned points (low 32 bits of the sum)
-    int pad;
+    int n_ordered;                 // work items appended by k_order_work
 };
 
 // Read-only view of a binned point set used by the traversal.
@@ -89,10 +90,12 @@ struct SetView {
     const void* coords;         // D planes of cap entries, cell-sorted
     int64_t cap;
     const DevStatus* st;
-    // SpatialHashingCellList only (else NULL), as of the last binning: per bucket the cell of
-    // its first point (3 int32) and whether its points lie in more than one cell
+    // SpatialHashingCellList only (else NULL), as of the last binning: per non-empty bucket the
+    // cell of one of its points (3 int32), and bucket_mixed[b] == bucket_epoch when its points
+    // lie in more than one cell (entries of earlier binnings hold earlier epochs)
     const int32_t* bucket_cell;
     const int32_t* bucket_mixed;
+    int bucket_epoch;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that each XCD
@@ -113,7 +116,7 @@ static __global__ void k_reset_status(DevStatus* st) {
     st->max_count = 0;
     st->n_nonempty = 0;
     st->n_binned = 0;
-    st->pad = 0;
+    st->n_ordered = 0;
 }
 
 static __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -205,15 +208,23 @@ __device__ __forceinline__ void hash_cell(const Geom& g, const TC* __restrict__ 
 }
 
 // spatial_hash (src/cell_lists/spatial_hashing.jl:150-164, Ihmsen et al. 2011) in Julia's
-// wrapping Int64 arithmetic, 0-based: mod(xor(i * 73856093, j * 19349663, k * 83492791), L).
+// wrapping Int64 arithmetic, 0-based: mod(xor(i * 73856093, j * 19349663, k * 83492791), L),
+// the floored mod of the signed hash.  For a negative hash h, mod(h, L) = L - 1 - (~h mod L)
+// (~h = -h - 1 >= 0).  x mod L for x < 2^63 is a Barrett reduction with
+// magic = floor((2^64 - 1) / L): q = mulhi(x, magic) > x / L - 3/2, so r = x - q L lies in
+// [0, 3L/2) and one conditional subtraction ends it (a 64-bit division is a long software
+// routine here).
 template <int D>
-__device__ __forceinline__ int spatial_hash(const int64_t (&c)[3], int list_size) {
+__device__ __forceinline__ int spatial_hash(const int64_t (&c)[3], const Geom& g) {
     uint64_t h = (uint64_t)c[0] * 73856093ull;
     if (D >= 2) h ^= (uint64_t)c[1] * 19349663ull;
     if (D >= 3) h ^= (uint64_t)c[2] * 83492791ull;
-    int64_t m = (int64_t)h % (int64_t)list_size;
-    if (m < 0) m += list_size;
-    return (int)m;
+    const uint64_t L = (uint64_t)g.list_size;
+    const bool neg = (int64_t)h < 0;
+    const uint64_t x = neg ? ~h : h;
+    uint64_t r = x - __umul64hi(x, g.list_magic) * L;
+    r = r >= L ? r - L : r;
+    return (int)(neg ? L - 1 - r : r);
 }
 
 __device__ __forceinline__ int64_t sel_point(int64_t a, int64_t sel_first, const int32_t* sel_list) {
@@ -238,7 +249,7 @@ __global__ __launch_bounds__(256) void k_bin(Geom g, const TC* __restrict__ y, i
         if (g.hash) {
             int64_t c[3];
             hash_cell<D, TC, TCELL>(g, y + (int64_t)D * p, c, ok);
-            k = spatial_hash<D>(c, g.list_size);
+            k = spatial_hash<D>(c, g);
         } else if (g.table) {
             const GeomEntry e = g.table[g.cloud_of_point[p]];
             k = e.cell_off + cell_key<D, TC, TCELL>(e.n, e.minc, g.csd, g, y + (int64_t)D * p, ok);
@@ -436,6 +447,40 @@ constexpr int CANON_RANK_LIMIT = 1024;
 // cell; neighbour sets are the same).  A wavefront's stores land inside the cell segments
 // of its 64 slots, so the permutation and the cell-sorted coordinate planes (plane d at
 // coords + d * cap) are written in whole lines.
+// Slot k's canonical position pos, its selection position a, point p, cell key and the first
+// slot s of the cell segment.
+struct CanonSlot {
+    int pos, key, s;
+    int32_t a, p;
+};
+__device__ __forceinline__ CanonSlot canon_slot(const int32_t* __restrict__ keys, int64_t sel_first,
+                                                const int32_t* __restrict__ sel_list,
+                                                const int32_t* __restrict__ start,
+                                                const int32_t* __restrict__ perm_tmp, int64_t k, int canonical) {
+    CanonSlot c;
+    c.a = perm_tmp[k];
+    c.key = keys[c.a];
+    c.p = (int32_t)sel_point(c.a, sel_first, sel_list);
+    c.s = start[c.key];
+    const int e = start[c.key + 1];
+    c.pos = (int)k;
+    if (canonical && e - c.s <= CANON_RANK_LIMIT) {
+        int rank = 0;
+        if (!sel_list) {
+            // original index = sel_first + selection position: rank by position
+            for (int q = c.s; q < e; ++q) rank += perm_tmp[q] < c.a;
+        } else {
+            for (int q = c.s; q < e; ++q) {
+                const int32_t aq = perm_tmp[q];
+                const int32_t pq = sel_list[aq];
+                rank += (pq < c.p) || (pq == c.p && aq < c.a);
+            }
+        }
+        c.pos = c.s + rank;
+    }
+    return c;
+}
+
 template <int D, class TC>
 __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys, int64_t sel_first,
                                                const int32_t* __restrict__ sel_list, const DevStatus* st,
@@ -445,61 +490,106 @@ __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
                                                TC* __restrict__ coords, int64_t cap, int canonical) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= st->n_binned) return;
-    const int32_t a = perm_tmp[k];
-    const int key = keys[a];
-    const int32_t p = (int32_t)sel_point(a, sel_first, sel_list);
-    const int s = start[key], e = start[key + 1];
-    int pos = (int)k;
-    if (canonical && e - s <= CANON_RANK_LIMIT) {
-        int rank = 0;
-        if (!sel_list) {
-            // original index = sel_first + selection position: rank by position
-            for (int q = s; q < e; ++q) rank += perm_tmp[q] < a;
-        } else {
-            for (int q = s; q < e; ++q) {
-                const int32_t aq = perm_tmp[q];
-                const int32_t pq = sel_list[aq];
-                rank += (pq < p) || (pq == p && aq < a);
-            }
-        }
-        pos = s + rank;
-    }
-    sorted_idx[pos] = p;
+    const CanonSlot c = canon_slot(keys, sel_first, sel_list, start, perm_tmp, k, canonical);
+    sorted_idx[c.pos] = c.p;
 #pragma unroll
-    for (int d = 0; d < D; ++d) coords[(int64_t)d * cap + pos] = y[(int64_t)D * p + d];
+    for (int d = 0; d < D; ++d) coords[(int64_t)d * cap + c.pos] = y[(int64_t)D * c.p + d];
 }
 
-// SpatialHashingCellList: per non-empty bucket, the cell of its first point and whether any
-// of its points lies in another cell (a hash collision), from the sorted coordinates of the
-// binning just done (the reference's collision flags, src/cell_lists/spatial_hashing.jl:
-// 84-116, computed once instead of per push).  One thread per non-empty bucket.
+// k_canon for a SpatialHashingCellList, which also records the bucket data the cell kernel
+// needs (the reference's collision flags, src/cell_lists/spatial_hashing.jl:84-116, computed
+// once per binning instead of per push): the point in the segment's first provisional slot s
+// gives the bucket's cell (written by slot s itself), and a point whose cell differs from it
+// marks the bucket as mixed (bucket_mixed[key] = epoch; every writer stores the same value).
 template <int D, class TC, class TCELL>
-__global__ __launch_bounds__(256) void k_bucket_info(Geom g, SetView S, int32_t* __restrict__ cell,
-                                                     int32_t* __restrict__ mixed) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= S.st->n_nonempty) return;
-    const int b = S.nonempty[t];
-    const int s = S.start[b], e = S.start[b + 1];
-    const TC* __restrict__ x = (const TC*)S.coords;
-    auto cell_of = [&](int k, int64_t (&c)[3]) {
-        TC p[D];
+__global__ __launch_bounds__(256) void k_canon_hash(const int32_t* __restrict__ keys, int64_t sel_first,
+                                                    const int32_t* __restrict__ sel_list, const DevStatus* st,
+                                                    const int32_t* __restrict__ start,
+                                                    const int32_t* __restrict__ perm_tmp,
+                                                    const TC* __restrict__ y, int32_t* __restrict__ sorted_idx,
+                                                    TC* __restrict__ coords, int64_t cap, int canonical, Geom g,
+                                                    int32_t* __restrict__ bucket_cell,
+                                                    int32_t* __restrict__ bucket_mixed, int epoch) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= st->n_binned) return;
+    const CanonSlot c = canon_slot(keys, sel_first, sel_list, start, perm_tmp, k, canonical);
+    TC x[D], xs[D];
+    const int64_t ps = sel_point(perm_tmp[c.s], sel_first, sel_list);
 #pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = x[(int64_t)d * S.cap + k];
-        bool ok;
-        hash_cell<D, TC, TCELL>(g, p, c, ok);
-    };
-    int64_t c0[3];
-    cell_of(s, c0);
-    int mx = 0;
-    for (int k = s + 1; k < e && !mx; ++k) {
-        int64_t c[3];
-        cell_of(k, c);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mx |= c[d] != c0[d];
+    for (int d = 0; d < D; ++d) {
+        x[d] = y[(int64_t)D * c.p + d];
+        xs[d] = y[(int64_t)D * ps + d];
     }
+    sorted_idx[c.pos] = c.p;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cell[3 * (int64_t)b + d] = (int32_t)c0[d];
-    mixed[b] = mx;
+    for (int d = 0; d < D; ++d) coords[(int64_t)d * cap + c.pos] = x[d];
+    int64_t ck[3], cs[3];
+    bool ok;
+    hash_cell<D, TC, TCELL>(g, x, ck, ok);
+    hash_cell<D, TC, TCELL>(g, xs, cs, ok);
+    if (k == c.s) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bucket_cell[3 * (int64_t)c.key + d] = (int32_t)cs[d];
+    }
+    bool same = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) same = same && ck[d] == cs[d];
+    if (!same) bucket_mixed[c.key] = epoch;
+}
+
+// SpatialHashingCellList work order: overwrites the non-empty bucket list with the buckets in
+// roughly the order of their first provisional points' selection positions.  Buckets sit at
+// hash positions, so bucket order scatters spatially adjacent work items over the XCDs; the
+// selection order follows the input's order, which is spatially coherent in every benchmark
+// (cell-sorted clouds), and keeps a neighbourhood's buckets in one XCD's L2.  A workgroup
+// compacts ORDER_ITEMS positions per thread in order and appends them with one atomic, so the
+// order across workgroups is their completion order; every work item writes only its own
+// queries' results, so results do not depend on it.
+constexpr int ORDER_THREADS = 1024, ORDER_ITEMS = 4;
+static __global__ __launch_bounds__(ORDER_THREADS) void k_order_work(const int32_t* __restrict__ keys,
+                                                                      const int32_t* __restrict__ start,
+                                                                      const int32_t* __restrict__ perm_tmp,
+                                                                      int64_t n_sel, DevStatus* st,
+                                                                      int32_t* __restrict__ nonempty) {
+    __shared__ int s_wave[ORDER_THREADS / WAVE];
+    __shared__ int s_base;
+    const int64_t a0 = ((int64_t)blockIdx.x * ORDER_THREADS + threadIdx.x) * ORDER_ITEMS;
+    int key[ORDER_ITEMS];
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < ORDER_ITEMS; ++i) {
+        const int64_t a = a0 + i;
+        key[i] = -1;
+        if (a < n_sel) {
+            const int kk = keys[a];
+            if (kk >= 0 && perm_tmp[start[kk]] == (int32_t)a) key[i] = kk;
+        }
+        n += key[i] >= 0;
+    }
+    // exclusive scan of n over the workgroup
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    int inc = n;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(inc, o, WAVE);
+        if (lane >= o) inc += t;
+    }
+    if (lane == WAVE - 1) s_wave[wv] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < ORDER_THREADS / WAVE; ++w) {
+            const int v = s_wave[w];
+            s_wave[w] = tot;
+            tot += v;
+        }
+        s_base = tot > 0 ? atomicAdd(&st->n_ordered, tot) : 0;
+    }
+    __syncthreads();
+    int pos = s_base + s_wave[wv] + inc - n;
+#pragma unroll
+    for (int i = 0; i < ORDER_ITEMS; ++i)
+        if (key[i] >= 0) nonempty[pos++] = key[i];
 }
 
 // Re-gathers the sorted coordinate planes from y (update-free coordinate refresh).

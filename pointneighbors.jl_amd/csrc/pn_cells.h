@@ -400,11 +400,11 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     if constexpr (PER) nc[d] = hash_wrap(nc[d], g.per[d]);
                     t /= 3;
                 }
-                const int b = spatial_hash<D>(nc, g.list_size);
+                const int b = spatial_hash<D>(nc, g);
                 rb = Gs.start[b];
                 rlen = Gs.start[b + 1] - rb;
                 if (rlen > 0) {
-                    if (Gs.bucket_mixed[b]) {
+                    if (Gs.bucket_mixed[b] == Gs.bucket_epoch) {
                         chk = true;
                     } else {
                         bool same = true;
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             set_pieces(rb, rlen);
         };
         if constexpr (HASH) {
-            qmixed = Q.bucket_mixed[qc] != 0;
+            qmixed = Q.bucket_mixed[qc] == Q.bucket_epoch;
             if (!qmixed) {
 #pragma unroll
                 for (int d = 0; d < D; ++d) hc[d] = Q.bucket_cell[3 * (int64_t)qc + d];

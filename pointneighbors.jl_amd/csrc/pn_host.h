@@ -97,6 +97,7 @@ struct Scanner {
 struct BinSet {
     Buf counts, start, nonempty, keys, slots, perm_tmp, sorted_idx, coords, status;
     Buf bucket_cell, bucket_mixed;  // SpatialHashingCellList: per-bucket cell and collision flag
+    int hash_epoch = 0;             // bucket_mixed value of a mixed bucket of the last binning
     pn::DevStatus* host_status = nullptr;  // pinned
     Scanner scan;
     int64_t cap = 0;
@@ -134,6 +135,7 @@ struct BinSet {
         v.st = status.as<pn::DevStatus>();
         v.bucket_cell = bucket_cell.as<int32_t>();
         v.bucket_mixed = bucket_mixed.as<int32_t>();
+        v.bucket_epoch = hash_epoch;
         return v;
     }
 };
@@ -233,7 +235,9 @@ int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
     if ((rc = s.status.ensure(sizeof(pn::DevStatus), g.stream))) return rc;
     if (g.geom.hash) {
         if ((rc = s.bucket_cell.ensure((size_t)C * 12, g.stream))) return rc;
+        const size_t before = s.bucket_mixed.bytes;
         if ((rc = s.bucket_mixed.ensure((size_t)C * 4, g.stream))) return rc;
+        if (s.bucket_mixed.bytes != before) s.hash_epoch = 0;  // new memory: cleared before use
     }
     if (!s.host_status) PN_HIP(hipHostMalloc((void**)&s.host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
     if (n_sel > s.cap) {
@@ -291,17 +295,32 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
                                s.perm_tmp.as<int32_t>());
             PN_CHECK_LAUNCH();
             // one thread per binned slot (at most sel_count; the count is read on the device)
-            hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
-                               s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
-                               s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
-                               s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order);
-            PN_CHECK_LAUNCH();
-            if (g.geom.hash) {
-                // at most min(buckets, points) non-empty buckets; the kernel reads the count
-                const int64_t nb = std::min<int64_t>(C, sel_count);
-                hipLaunchKernelGGL((pn::k_bucket_info<D, TC, TCELL>), dim3(blocks_for(nb, 256)), dim3(256), 0,
-                                   g.stream, g.geom, s.view(), s.bucket_cell.as<int32_t>(),
-                                   s.bucket_mixed.as<int32_t>());
+            if (!g.geom.hash) {
+                hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
+                                   s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
+                                   s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
+                                   s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order);
+                PN_CHECK_LAUNCH();
+            } else {
+                // mixed-bucket flags of earlier binnings hold earlier epochs; fresh memory (epoch 0)
+                // and a wrapped counter clear the array first
+                if (s.hash_epoch <= 0 || s.hash_epoch == INT32_MAX) {
+                    hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(C, 1 << 20), 256)),
+                                       dim3(256), 0, g.stream, s.bucket_mixed.as<int32_t>(), C, 0);
+                    PN_CHECK_LAUNCH();
+                    s.hash_epoch = 0;
+                }
+                ++s.hash_epoch;
+                hipLaunchKernelGGL((pn::k_canon_hash<D, TC, TCELL>), dim3(blocks_for(sel_count, 256)), dim3(256), 0,
+                                   g.stream, s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
+                                   s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
+                                   s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order, g.geom,
+                                   s.bucket_cell.as<int32_t>(), s.bucket_mixed.as<int32_t>(), s.hash_epoch);
+                PN_CHECK_LAUNCH();
+                const int64_t per_block = (int64_t)pn::ORDER_THREADS * pn::ORDER_ITEMS;
+                hipLaunchKernelGGL(pn::k_order_work, dim3((unsigned)((sel_count + per_block - 1) / per_block)),
+                                   dim3(pn::ORDER_THREADS), 0, g.stream, s.keys.as<int32_t>(), s.start.as<int32_t>(),
+                                   s.perm_tmp.as<int32_t>(), sel_count, st, s.nonempty.as<int32_t>());
                 PN_CHECK_LAUNCH();
             }
         }

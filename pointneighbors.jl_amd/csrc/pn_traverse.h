@@ -179,7 +179,7 @@ __device__ __forceinline__ void hash_query(const Geom& g, const SetView& Q, cons
             if constexpr (PER) nc[d] = hash_wrap(nc[d], g.per[d]);
             t /= 3;
         }
-        const int b = spatial_hash<D>(nc, g.list_size);
+        const int b = spatial_hash<D>(nc, g);
         const int s1 = G.start[b + 1];
         for (int src = G.start[b]; src < s1; ++src) {
             TC xj[D];
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
                 if constexpr (HPER) nc[d] = hash_wrap(nc[d], g.per[d]);
                 t /= 3;
             }
-            const int b = spatial_hash<D>(nc, g.list_size);
+            const int b = spatial_hash<D>(nc, g);
             rb = G.start[b];
             rlen = G.start[b + 1] - rb;
         } else {

@@ -284,8 +284,16 @@ int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype
  *                      replay them instead of testing candidates, with identical results.
  *                      0 (default): always test.  The adjoints always recompute the traversal.
  *   "pair_cache_per_point"  cache capacity in 2-byte entries per point (0: 108 in 3D, 36 in
- *                      2D, 12 in 1D); cells that do not fit are traversed in full */
+ *                      2D, 12 in 1D); cells that do not fit are traversed in full
+ *   "time_launches"    1: record a HIP event pair on the grid's stream around every traversal
+ *                      kernel from now on (at most 4096 until read), 0: stop; either clears
+ *                      the records */
 int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value);
+
+/* Measurement hook, no reference counterpart: waits for the last recorded launch, writes the
+ * durations in ms of the traversal kernels recorded since "time_launches" was set (at most
+ * max, in launch order), their number to *n, and clears the records. */
+int pn_grid_launch_times(pn_grid_t grid, float* ms, int64_t max, int64_t* n);
 
 /* Adds U[-amplitude, amplitude] to each of the n values of x (a pure function of seed,
  * *step and the index), then increments *step on the device: a replayable per-step

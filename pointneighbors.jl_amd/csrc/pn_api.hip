@@ -330,6 +330,17 @@ int pn_batch_create(int ndims, int coord_dtype, int radius_dtype, int cell_dtype
     return PN_OK;
 }
 
+int pn_grid_launch_times(pn_grid_t grid, float* ms, int64_t max, int64_t* n) {
+    if (!grid || !n || (max > 0 && !ms)) return fail(PN_ERR_ARGUMENT, "NULL pointer");
+    const int64_t pairs = (int64_t)(grid->timing_used / 2);
+    if (pairs > 0) PN_HIP(hipEventSynchronize(grid->timing_events[grid->timing_used - 1]));
+    for (int64_t i = 0; i < pairs && i < max; ++i)
+        PN_HIP(hipEventElapsedTime(&ms[i], grid->timing_events[2 * i], grid->timing_events[2 * i + 1]));
+    *n = pairs;
+    grid->timing_used = 0;
+    return PN_OK;
+}
+
 int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value) {
     if (!grid || !key) return fail(PN_ERR_ARGUMENT, "NULL pointer");
     const std::string k(key);
@@ -348,6 +359,12 @@ int pn_set_tuning(pn_grid_t grid, const char* key, int64_t value) {
         if (value != 0 && value != 1) return fail(PN_ERR_ARGUMENT, "algorithm must be 0 or 1");
         grid->algorithm = (int)value;
         grid->pc_gen = ~0ull;
+        return PN_OK;
+    }
+    if (k == "time_launches") {
+        if (value != 0 && value != 1) return fail(PN_ERR_ARGUMENT, "time_launches must be 0 or 1");
+        grid->time_launches = (int)value;
+        grid->timing_used = 0;
         return PN_OK;
     }
     if (k == "pair_cache") {

@@ -501,6 +501,8 @@ __global__ __launch_bounds__(256) void k_canon(const int32_t* __restrict__ keys,
 // once per binning instead of per push): the point in the segment's first provisional slot s
 // gives the bucket's cell (written by slot s itself), and a point whose cell differs from it
 // marks the bucket as mixed (bucket_mixed[key] = epoch; every writer stores the same value).
+// Slot s also names its selection position a as the bucket's representative: rep[a] = key
+// (rep holds -1 elsewhere).
 template <int D, class TC, class TCELL>
 __global__ __launch_bounds__(256) void k_canon_hash(const int32_t* __restrict__ keys, int64_t sel_first,
                                                     const int32_t* __restrict__ sel_list, const DevStatus* st,
@@ -509,7 +511,8 @@ __global__ __launch_bounds__(256) void k_canon_hash(const int32_t* __restrict__ 
                                                     const TC* __restrict__ y, int32_t* __restrict__ sorted_idx,
                                                     TC* __restrict__ coords, int64_t cap, int canonical, Geom g,
                                                     int32_t* __restrict__ bucket_cell,
-                                                    int32_t* __restrict__ bucket_mixed, int epoch) {
+                                                    int32_t* __restrict__ bucket_mixed, int epoch,
+                                                    int32_t* __restrict__ rep) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= st->n_binned) return;
     const CanonSlot c = canon_slot(keys, sel_first, sel_list, start, perm_tmp, k, canonical);
@@ -530,6 +533,7 @@ __global__ __launch_bounds__(256) void k_canon_hash(const int32_t* __restrict__ 
     if (k == c.s) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) bucket_cell[3 * (int64_t)c.key + d] = (int32_t)cs[d];
+        rep[c.a] = c.key;  // the bucket's representative selection position (for k_order_work)
     }
     bool same = true;
 #pragma unroll
@@ -538,18 +542,16 @@ __global__ __launch_bounds__(256) void k_canon_hash(const int32_t* __restrict__ 
 }
 
 // SpatialHashingCellList work order: overwrites the non-empty bucket list with the buckets in
-// roughly the order of their first provisional points' selection positions.  Buckets sit at
-// hash positions, so bucket order scatters spatially adjacent work items over the XCDs; the
+// the order of their representatives' selection positions (rep[a] >= 0, from k_canon_hash),
+// roughly: a workgroup compacts ORDER_ITEMS positions per thread in order and appends them with
+// one atomic, so the order across workgroups is their completion order.  Buckets sit at hash
+// positions, so bucket order scatters spatially adjacent work items over the XCDs; the
 // selection order follows the input's order, which is spatially coherent in every benchmark
-// (cell-sorted clouds), and keeps a neighbourhood's buckets in one XCD's L2.  A workgroup
-// compacts ORDER_ITEMS positions per thread in order and appends them with one atomic, so the
-// order across workgroups is their completion order; every work item writes only its own
-// queries' results, so results do not depend on it.
+// (cell-sorted clouds), and keeps a neighbourhood's buckets in one XCD's L2.  Every work item
+// writes only its own queries' results, so results do not depend on the order.
 constexpr int ORDER_THREADS = 1024, ORDER_ITEMS = 4;
-static __global__ __launch_bounds__(ORDER_THREADS) void k_order_work(const int32_t* __restrict__ keys,
-                                                                      const int32_t* __restrict__ start,
-                                                                      const int32_t* __restrict__ perm_tmp,
-                                                                      int64_t n_sel, DevStatus* st,
+static __global__ __launch_bounds__(ORDER_THREADS) void k_order_work(const int32_t* __restrict__ rep, int64_t n_sel,
+                                                                      DevStatus* st,
                                                                       int32_t* __restrict__ nonempty) {
     __shared__ int s_wave[ORDER_THREADS / WAVE];
     __shared__ int s_base;
@@ -559,11 +561,7 @@ static __global__ __launch_bounds__(ORDER_THREADS) void k_order_work(const int32
 #pragma unroll
     for (int i = 0; i < ORDER_ITEMS; ++i) {
         const int64_t a = a0 + i;
-        key[i] = -1;
-        if (a < n_sel) {
-            const int kk = keys[a];
-            if (kk >= 0 && perm_tmp[start[kk]] == (int32_t)a) key[i] = kk;
-        }
+        key[i] = a < n_sel ? rep[a] : -1;
         n += key[i] >= 0;
     }
     // exclusive scan of n over the workgroup

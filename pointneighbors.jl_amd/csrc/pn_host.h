@@ -173,8 +173,14 @@ struct pn_grid {
     int64_t err_point = -1;
     int64_t err_cell[3] = {0, 0, 0};
     const void* err_src = nullptr;
+    // traversal kernel timing (tuning key "time_launches"): events 2i and 2i + 1 bracket the
+    // i-th traversal kernel launched since timing was switched on
+    int time_launches = 0;
+    std::vector<hipEvent_t> timing_events;
+    size_t timing_used = 0;
 
     ~pn_grid() {
+        for (hipEvent_t e : timing_events) (void)hipEventDestroy(e);
         fields.release();
         pc_data.release();
         pc_head.release();
@@ -255,6 +261,35 @@ int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
     return PN_OK;
 }
 
+// Records a HIP event on the grid's stream before and after the traversal kernel launched in
+// its scope, when "time_launches" is on (at most MAX_TIMED launches until they are read).
+struct LaunchTimer {
+    static constexpr size_t MAX_TIMED = 4096;
+    pn_grid& g;
+    bool on;
+    explicit LaunchTimer(pn_grid& grid) : g(grid), on(grid.time_launches != 0) {
+        if (!on) return;
+        if (g.timing_used + 2 > 2 * MAX_TIMED) {
+            on = false;
+            return;
+        }
+        while (g.timing_events.size() < g.timing_used + 2) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                on = false;
+                return;
+            }
+            g.timing_events.push_back(e);
+        }
+        (void)hipEventRecord(g.timing_events[g.timing_used], g.stream);
+    }
+    ~LaunchTimer() {
+        if (!on) return;
+        (void)hipEventRecord(g.timing_events[g.timing_used + 1], g.stream);
+        g.timing_used += 2;
+    }
+};
+
 unsigned blocks_for(int64_t n, int threads) { return (unsigned)std::max<int64_t>(1, (n + threads - 1) / threads); }
 
 // Enqueues the counting-sort binning of the selected points of src into s.
@@ -311,16 +346,21 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
                     s.hash_epoch = 0;
                 }
                 ++s.hash_epoch;
+                // bucket representatives by selection position, in `slots` (free after k_scatter)
+                int32_t* rep = s.slots.as<int32_t>();
+                hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(sel_count, 1 << 20), 256)),
+                                   dim3(256), 0, g.stream, rep, sel_count, -1);
+                PN_CHECK_LAUNCH();
                 hipLaunchKernelGGL((pn::k_canon_hash<D, TC, TCELL>), dim3(blocks_for(sel_count, 256)), dim3(256), 0,
                                    g.stream, s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
                                    s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,
                                    s.sorted_idx.as<int32_t>(), s.coords.as<TC>(), s.cap, g.canonical_order, g.geom,
-                                   s.bucket_cell.as<int32_t>(), s.bucket_mixed.as<int32_t>(), s.hash_epoch);
+                                   s.bucket_cell.as<int32_t>(), s.bucket_mixed.as<int32_t>(), s.hash_epoch, rep);
                 PN_CHECK_LAUNCH();
                 const int64_t per_block = (int64_t)pn::ORDER_THREADS * pn::ORDER_ITEMS;
                 hipLaunchKernelGGL(pn::k_order_work, dim3((unsigned)((sel_count + per_block - 1) / per_block)),
-                                   dim3(pn::ORDER_THREADS), 0, g.stream, s.keys.as<int32_t>(), s.start.as<int32_t>(),
-                                   s.perm_tmp.as<int32_t>(), sel_count, st, s.nonempty.as<int32_t>());
+                                   dim3(pn::ORDER_THREADS), 0, g.stream, (const int32_t*)rep, sel_count, st,
+                                   s.nonempty.as<int32_t>());
                 PN_CHECK_LAUNCH();
             }
         }
@@ -460,6 +500,7 @@ int launch_traverse_t(pn_grid& g, const BinSet* Q, F f, const pn::FieldSrc<TR>* 
     const int nblocks = (int)std::min<int64_t>(round_up8(nwork), g.persist_blocks);
     const TR r = (TR)g.radius;
     const TR r2 = r * r;
+    LaunchTimer timer(g);  // brackets the one traversal kernel launched below
     if (g.geom.hash && (g.algorithm == 1 || !(Q->fresh && g.grid.fresh))) {
         // a SpatialHashingCellList whose coordinates moved since the binning (the per-bucket
         // cells of the binning no longer hold: every candidate's cell is recomputed), or the

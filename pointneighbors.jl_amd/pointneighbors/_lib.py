@@ -88,6 +88,7 @@ SIGNATURES = {
     "pn_batch_create": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_i64, _p_dbl, _p_i64, _c_dbl, _c_i32,
                                  _p_i64, ctypes.POINTER(_vp)]),
     "pn_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_i64]),
+    "pn_grid_launch_times": (_c_int, [_vp, ctypes.POINTER(ctypes.c_float), _c_i64, _p_i64]),
     "pn_grid_set_periodic": (_c_int, [_vp, _p_dbl, _p_i64, _p_dbl]),
     "pn_displace_uniform": (_c_int, [_vp, _c_i64, _c_int, _c_dbl, ctypes.c_uint64, _vp, _vp]),
     "pn_list_sph_density": (_c_int, [_vp, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp,

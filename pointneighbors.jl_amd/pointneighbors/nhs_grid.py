@@ -426,6 +426,15 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             out.extend(idx[start[c]:start[c + 1]].tolist())
         return out
 
+    def launch_times(self, max_launches=4096):
+        """Durations (ms) of the traversal kernels launched since set_tuning("time_launches", 1),
+        in launch order, from HIP events the library records on its stream around each launch;
+        waits for the last one and clears the records."""
+        buf = (ctypes.c_float * max_launches)()
+        n = ctypes.c_int64()
+        check(load().pn_grid_launch_times(self._handle, buf, max_launches, ctypes.byref(n)), "launch_times")
+        return [buf[i] for i in range(min(n.value, max_launches))]
+
     def stats(self):
         lib = load()
         nb, ne, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

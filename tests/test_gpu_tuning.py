@@ -87,3 +87,24 @@ def test_autotune_picks_and_caches(tmp_path):
                      pts.max(axis=0) + 3 * nhs.search_radius)
     assert autotune.load_tuning(other, cache, key)
     assert other._tuning["persist_blocks"] == best["persist_blocks"]
+
+
+def test_launch_timing_hook():
+    """pn_grid_launch_times: one positive duration per traversal kernel launched while
+    "time_launches" is on, in launch order, cleared on read; none when it is off; results
+    unchanged."""
+    pts, h, nhs, x = setup()
+    c0, rho0 = run(nhs, x, h)
+    nhs.set_tuning("time_launches", 1)
+    c1, rho1 = run(nhs, x, h)  # a count and a density traversal
+    c2, rho2 = run(nhs, x, h)
+    t = nhs.launch_times()
+    assert len(t) == 4 and all(v > 0 for v in t)
+    assert nhs.launch_times() == []
+    nhs.set_tuning("time_launches", 0)
+    run(nhs, x, h)
+    assert nhs.launch_times() == []
+    assert np.array_equal(c0, c1) and np.array_equal(c1, c2)
+    assert np.array_equal(rho0, rho1) and np.array_equal(rho1, rho2)
+    with pytest.raises(pn.ArgumentError):
+        nhs.set_tuning("time_launches", 2)

@@ -177,10 +177,12 @@ struct MapReduceF : FunctorBase {
         else if (fn == MR_DISTANCE) val = (double)sqrt(d2);  // IEEE sqrt in the radius type
         else if (fn == MR_DISTANCE2) val = (double)d2;
         else {
-            T c = pd[0];
-#pragma unroll
-            for (int d = 1; d < DD; ++d) c = fn - MR_POSDIFF == d ? pd[d] : c;
-            val = (double)c;
+            // the components pass through an empty asm first: a selection among plain loads of
+            // pd was folded into one load at a selected offset, which put pd in scratch memory
+            T p0 = pd[0], p1 = pd[DD > 1 ? 1 : 0], p2 = pd[DD > 2 ? 2 : 0];
+            asm("" : "+v"(p0), "+v"(p1), "+v"(p2));
+            const int k = fn - MR_POSDIFF;
+            val = (double)(k == 2 ? p2 : (k == 1 ? p1 : p0));
         }
         a.v = apply(a.v, val);
         a.n += 1;

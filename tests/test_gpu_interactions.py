@@ -119,6 +119,36 @@ def test_wcsph_interaction_rejects_other_queries(orc):
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_mapreduce_posdiff_components_3d(orc, dt):
+    """Each pos_diff component in 3D (the component is selected per call): min and max are
+    exact, sums within the summation-order tolerance."""
+    pts = synthetic.jittered_lattice((11, 12, 13), 1.0, 0.3, dtype=dt, seed=23)
+    r = dt(2.2)
+    nhs, x, lists = grid_and_lists(orc, pts, r)
+    rp, cols = lists
+    n = len(pts)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    pd = pts[rows].astype(dt) - pts[cols].astype(dt)
+    for c in range(3):
+        vals = pd[:, c].astype(np.float64)
+        lo = np.full(n, 1e9)
+        np.minimum.at(lo, rows, vals)
+        hi = np.full(n, -1e9)
+        np.maximum.at(hi, rows, vals)
+        sm = np.zeros(n)
+        np.add.at(sm, rows, vals)
+        ab = np.zeros(n)
+        np.add.at(ab, rows, np.abs(vals))
+        got_lo, _ = pn.mapreduce_point_neighbors(pn.PosDiff(c), min, x, x, nhs, init=1e9)
+        got_hi, _ = pn.mapreduce_point_neighbors(pn.PosDiff(c), max, x, x, nhs, init=-1e9)
+        got_sm, _ = pn.mapreduce_point_neighbors(pn.PosDiff(c), operator.add, x, x, nhs, init=0.0)
+        assert np.array_equal(got_lo.cpu().numpy(), lo), c
+        assert np.array_equal(got_hi.cpu().numpy(), hi), c
+        tol = (1e-5 if dt == np.float32 else 1e-12) * ab
+        assert (np.abs(got_sm.cpu().numpy() - sm) <= tol).all(), c
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_mapreduce_menu(orc, dt):
     """Per-point reductions against the oracle's neighbour lists: counts, index sums, distances
     (sum, min, max), squared distances and pos_diff components; init for an empty

@@ -92,9 +92,9 @@ using IntC = std::integral_constant<int, N>;
 
 // 64 / n for n = 0 .. 64 (0 -> 64): the lanes per query of a flush, read with a scalar load.
 struct LanesTable {
-    uint8_t v[WAVE + 1];
+    int v[WAVE + 1];  // words: a byte table is read with a vector load
     constexpr LanesTable() : v{} {
-        for (int n = 0; n <= WAVE; ++n) v[n] = (uint8_t)(WAVE / (n > 0 ? n : 1));
+        for (int n = 0; n <= WAVE; ++n) v[n] = WAVE / (n > 0 ? n : 1);
     }
 };
 static __constant__ constexpr LanesTable kLanesTable{};
@@ -156,7 +156,16 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     // (larger states would cost occupancy).
     constexpr bool kQstLds = sizeof(QS) <= 4 || F::kFill;
     __shared__ QS s_qst[kQstLds ? LQ : 1];
-    __shared__ Acc s_acc[LQ];
+    // the batch's per-query sums: in a register of the query's lane (lane k holds query k's;
+    // frees LDS, hence occupancy) for sums of up to three words or in Float64 (measured faster),
+    // else in LDS
+    constexpr bool kRegAcc = sizeof(Acc) <= 12 || sizeof(TR) == 8;
+    Acc qacc = f.init();
+    __shared__ Acc s_acc[(!kLight && !kRegAcc) ? LQ : 1];
+    auto query_sum = [&]() -> Acc {
+        if constexpr (kRegAcc) return qacc;
+        else return s_acc[threadIdx.x];
+    };
     __shared__ int s_written[F::kFill ? WAVE : 1];
 
     const int lane = threadIdx.x;
@@ -205,8 +214,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     // Runs the pair body over pairs [0, P) of list `prp`, which hold queries i0 .. i1-1
     // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Each query gets
     // L = 64 / (i1 - i0) lanes, which take its pairs round-robin; the L partial sums are
-    // combined by a tree of fixed order and added to s_acc by the query's first lane, so sums
-    // are deterministic.  The pair bodies read a query's coordinates from its sorted plane
+    // combined by a tree of fixed order and then added, in flush order, to the query's sum, so
+    // sums are deterministic.  The pair bodies read a query's coordinates from its sorted plane
     // (L1/L2-resident; keeping them out of LDS buys occupancy).
     int cur_q0 = 0;  // sorted position of the current query batch
     auto qpack = [&](int q) { return s_qc[q]; };
@@ -306,10 +315,16 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             const Acc t = shfl_acc(a, lane + o);
             if (sub < o) f.combine(a, t);
         }
-        if (q < nf && sub == 0) {
-            Acc t = s_acc[i0 + q];
-            f.combine(t, a);
-            s_acc[i0 + q] = t;
+        if constexpr (kRegAcc) {
+            // lane k (query k of the batch) takes its flush sum from lane (k - i0) L
+            const Acc t = shfl_acc(a, (lane - i0) * L);
+            if (lane >= i0 && lane < i1) f.combine(qacc, t);
+        } else {
+            if (q < nf && sub == 0) {
+                Acc t = s_acc[i0 + q];
+                f.combine(t, a);
+                s_acc[i0 + q] = t;
+            }
         }
         __syncthreads();
     };
@@ -609,7 +624,8 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             if constexpr (!kLight) {
                 if (qact) {
                     if constexpr (kQstLds) s_qst[lane] = f.query(orig_i);
-                    s_acc[lane] = f.init();
+                    if constexpr (kRegAcc) qacc = f.init();
+                    else s_acc[lane] = f.init();
                     if constexpr (F::kFill) s_written[lane] = 0;
                 }
             }
@@ -649,7 +665,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                     pos += 5 + nf + P;
                 }
                 __syncthreads();
-                if (lane < nq) f.write(orig_i, qstate(lane), s_acc[lane]);
+                if (lane < nq) f.write(orig_i, qstate(lane), query_sum());
             }
             continue;
         }
@@ -782,7 +798,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                 if (qact) f.write(orig_i, QS{}, mycount);
             } else {
                 __syncthreads();
-                if (qact) f.write(orig_i, qstate(lane), s_acc[lane]);
+                if (qact) f.write(orig_i, qstate(lane), query_sum());
             }
 
         }

@@ -37,6 +37,13 @@ def clouds():
     sp = F32(0.01)
     yield ("WCSPH-like jittered lattice", synthetic.jittered_lattice((30, 31, 32), 0.01, 0.001, dtype=F32),
            F32(2) * (F32(1.2) * sp))
+    # extreme scales: lattices with near-ties scaled to tiny and huge coordinates and radii
+    # (r^2 near 1e-31 and terms near 1e30 in Float32)
+    base = synthetic.jittered_lattice((12, 13, 14), 1.0, 0.0, dtype=np.float64)
+    jit = rng.uniform(-0.02, 0.02, size=base.shape)
+    for scale, label in ((1e-16, "tiny scale, r^2 below 2^-100"), (1e-14, "small scale, band test"),
+                         (1e13, "large scale, band test"), (1e14, "huge scale, terms above 2^100")):
+        yield label, ((base + jit) * scale).astype(F32), F32(3.0 * scale)
 
 
 CASES = list(clouds())

@@ -148,9 +148,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     __shared__ uint64_t s_pairs64[kLight ? 1 : PAIR_WORDS];
     PairT* const s_pairs = (PairT*)s_pairs64;
     __shared__ uint16_t s_qstart[LQ + 1];
-    // the query batch's coordinates: read with one broadcast LDS load per query in the test
-    // loop and per lane in the pair bodies
-    __shared__ Pack<TC, D> s_qc[WAVE];
+    // the query batch's coordinates, lane k holding query k's: read with one broadcast LDS load
+    // per query in the test loop and per lane in the pair bodies; in Float64 pair functors they
+    // are taken from the query's lane instead (a readlane per query, a shuffle per flush), which
+    // keeps 2 KB of LDS free for occupancy (measured faster there, slower in Float32)
+    constexpr bool kQcLds = kLight || sizeof(TC) == 4;
+    __shared__ Pack<TC, D> s_qc[kQcLds ? WAVE : 1];
+    TC qxr[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) qxr[d] = (TC)0;
     // Per-query state (f.query): kept in LDS when it is one word or the CSR fill's row bounds
     // (read at almost every pair), else reloaded from global memory where a flush needs it
     // (larger states would cost occupancy).
@@ -215,10 +221,21 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
     // (qsp[q - i0] = first pair of query q, qsp[i1 - i0] = P).  Each query gets
     // L = 64 / (i1 - i0) lanes, which take its pairs round-robin; the L partial sums are
     // combined by a tree of fixed order and then added, in flush order, to the query's sum, so
-    // sums are deterministic.  The pair bodies read a query's coordinates from its sorted plane
-    // (L1/L2-resident; keeping them out of LDS buys occupancy).
+    // sums are deterministic.
     int cur_q0 = 0;  // sorted position of the current query batch
-    auto qpack = [&](int q) { return s_qc[q]; };
+    // query q's coordinates (every lane calls it: the shuffle reads lane q)
+    auto qpack = [&](int q) {
+        Pack<TC, D> p;
+        if constexpr (kQcLds) {
+            p = s_qc[q];
+        } else {
+#pragma unroll
+            for (int d = 0; d < (D == 3 ? 4 : D); ++d) p.v[d] = (TC)0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) p.v[d] = shfl_acc(qxr[d], q);
+        }
+        return p;
+    };
     // ... and its per-query state.
     auto qstate = [&](int q) -> QS {
         if constexpr (kQstLds) return s_qst[q];
@@ -268,15 +285,15 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             __syncthreads();
             return;
         }
-        // L lanes per query (the largest power of two with L * nf <= 64): lane l takes query
-        // q = l / L and its pairs qsp[q] + (l % L) + L t, so no lane changes query inside the
-        // loop; the L partial sums of a query are then combined by a butterfly of fixed order.
+        // L = floor(64 / nf) lanes per query: lane l takes query q = l / L and its pairs
+        // qsp[q] + (l % L) + L t, so no lane changes query inside the loop; the L partial sums
+        // of a query are then combined by a tree of fixed order.
         const int L = kLanesTable.v[nf];
         // lane / L exactly: (lane + 1/2) / L is at least 1/(2L) away from an integer
         const int q = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)L)), sub = lane - q * L;
         Acc a = f.init();
+        const Pack<TC, D> qp = qpack(i0 + q);
         if (q < nf) {
-            const Pack<TC, D> qp = qpack(i0 + q);
             const QS qst = qstate(i0 + q);
             const int ke = qsp[q + 1];
             for (int k = qsp[q] + sub; k < ke; k += L) {
@@ -614,11 +631,14 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
             for (int d = 0; d < D; ++d) xq[d] = qact ? qcrd[(int64_t)d * Q.cap + q0 + lane] : (TC)0;
             orig_i = qact ? Q.sorted_idx[q0 + lane] : 0;
             __syncthreads();  // the previous batch is done with s_qc
-            {
+            if constexpr (kQcLds) {
                 Pack<TC, D> pk;
 #pragma unroll
                 for (int d = 0; d < (D == 3 ? 4 : D); ++d) pk.v[d] = d < D ? xq[d] : (TC)0;
                 s_qc[lane] = pk;
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; ++d) qxr[d] = xq[d];
             }
             __syncthreads();
             if constexpr (!kLight) {
@@ -773,7 +793,7 @@ __global__ __launch_bounds__(WAVE) void k_cells(Geom g, SetView Q, SetView Gs, T
                             }
                             TC xs[D];
 #pragma unroll
-                            for (int d = 0; d < D; ++d) xs[d] = s_qc[i].v[d];
+                            for (int d = 0; d < D; ++d) xs[d] = kQcLds ? s_qc[i].v[d] : lane_bcast(qxr[d], i);
 #pragma unroll
                             for (int gi = 0; gi < NG; ++gi) {
                                 // accepted lanes take the next list slots in lane order (the

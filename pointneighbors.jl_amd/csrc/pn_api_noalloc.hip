@@ -152,7 +152,8 @@ int pn_count_neighbors_noalloc(int ndims, int coord_dtype, int radius_dtype, int
         hipLaunchKernelGGL(pn::k_scan_reduce, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, s, (const int32_t*)cnt,
                            C, bs, st);
         PN_CHECK_LAUNCH();
-        hipLaunchKernelGGL(pn::k_scan_top, dim3(1), dim3(pn::SCAN_THREADS), 0, s, bs, nb, start, C, st);
+        hipLaunchKernelGGL(pn::k_scan_top, dim3(1), dim3(pn::SCAN_THREADS), 0, s, bs, nb, start, C, st,
+                           (pn::StatusMailbox*)nullptr, 0ull);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scan_down, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, s, (const int32_t*)cnt, C,
                            (const unsigned long long*)bs, start, nonempty);

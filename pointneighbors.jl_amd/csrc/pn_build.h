@@ -82,6 +82,15 @@ struct DevStatus {
     int n_ordered;                 // work items appended by k_order_work
 };
 
+// Host copy of a binning's status in pinned, coherent host memory.  k_scan_top, the last
+// kernel of a binning to change the fields the host checks, stores the status there and
+// then raises `seq` with a system-scope release, so the host can raise the binning's
+// errors while the scatter and the in-cell ordering still run.
+struct StatusMailbox {
+    DevStatus status;
+    unsigned long long seq;
+};
+
 // Read-only view of a binned point set used by the traversal.
 struct SetView {
     const int32_t* start;       // C + 1 cell offsets
@@ -350,7 +359,8 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32
 // block_sums[nblocks, 2 nblocks)).
 static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long long* __restrict__ block_sums,
                                                            int64_t nblocks, int32_t* __restrict__ start,
-                                                           int64_t C, DevStatus* st) {
+                                                           int64_t C, DevStatus* st, StatusMailbox* mb,
+                                                           unsigned long long seq) {
     __shared__ unsigned long long sh[SCAN_THREADS / WAVE];
     __shared__ unsigned long long sh_sum[SCAN_THREADS / WAVE];
     __shared__ int sh_max[SCAN_THREADS / WAVE];
@@ -391,6 +401,16 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long 
         start[C] = (int32_t)(uint32_t)(carry & 0xffffffffull);
         st->n_binned = (int)(uint32_t)(carry & 0xffffffffull);
         st->n_nonempty = (int)(uint32_t)(carry >> 32);
+        if (mb) {
+            // first_bad comes from k_bin, an earlier kernel on the same stream
+            mb->status.first_bad = st->first_bad;
+            mb->status.total64 = tot;
+            mb->status.max_count = m;
+            mb->status.n_nonempty = (int)(uint32_t)(carry >> 32);
+            mb->status.n_binned = (int)(uint32_t)(carry & 0xffffffffull);
+            mb->status.n_ordered = 0;
+            __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 

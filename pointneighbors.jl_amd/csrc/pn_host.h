@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -76,7 +77,7 @@ struct Buf {
 struct Scanner {
     Buf block_sums;
     int run(const int32_t* counts, int64_t n, int32_t* start, int32_t* nonempty, pn::DevStatus* st,
-            hipStream_t stream) {
+            hipStream_t stream, pn::StatusMailbox* mb = nullptr, unsigned long long seq = 0) {
         const int64_t nb = std::max<int64_t>(1, (n + pn::SCAN_TILE - 1) / pn::SCAN_TILE);
         int rc = block_sums.ensure((size_t)nb * 16, stream);  // block sums, then block maxima
         if (rc) return rc;
@@ -84,7 +85,8 @@ struct Scanner {
         hipLaunchKernelGGL(pn::k_scan_reduce, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts,
                            n, bs, st);
         PN_CHECK_LAUNCH();
-        hipLaunchKernelGGL(pn::k_scan_top, dim3(1), dim3(pn::SCAN_THREADS), 0, stream, bs, nb, start, n, st);
+        hipLaunchKernelGGL(pn::k_scan_top, dim3(1), dim3(pn::SCAN_THREADS), 0, stream, bs, nb, start, n, st, mb,
+                           seq);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scan_down, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts, n,
                            (const unsigned long long*)bs, start, nonempty);
@@ -99,6 +101,11 @@ struct BinSet {
     Buf bucket_cell, bucket_mixed;  // SpatialHashingCellList: per-bucket cell and collision flag
     int hash_epoch = 0;             // bucket_mixed value of a mixed bucket of the last binning
     pn::DevStatus* host_status = nullptr;  // pinned
+    // pinned coherent host memory k_scan_top publishes the status to; mail_seq is the
+    // sequence number of the last binning that asked for it
+    pn::StatusMailbox* mailbox = nullptr;
+    unsigned long long mail_seq = 0;
+    bool mail_pending = false;
     Scanner scan;
     int64_t cap = 0;
     int64_t n_sel = 0;        // selected points of the last binning
@@ -123,6 +130,7 @@ struct BinSet {
         bucket_mixed.release();
         scan.block_sums.release();
         if (host_status) (void)hipHostFree(host_status);
+        if (mailbox) (void)hipHostFree(mailbox);
     }
 
     pn::SetView view() const {
@@ -246,6 +254,12 @@ int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
         if (s.bucket_mixed.bytes != before) s.hash_epoch = 0;  // new memory: cleared before use
     }
     if (!s.host_status) PN_HIP(hipHostMalloc((void**)&s.host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
+    if (!s.mailbox) {
+        PN_HIP(hipHostMalloc((void**)&s.mailbox, sizeof(pn::StatusMailbox),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset((void*)s.mailbox, 0, sizeof(pn::StatusMailbox));
+        s.mail_seq = 0;
+    }
     if (n_sel > s.cap) {
         int64_t cap = std::max<int64_t>(n_sel, 64);
         if ((rc = s.keys.ensure((size_t)cap * 4, g.stream))) return rc;
@@ -306,6 +320,8 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
     if (rc) return rc;
     const int64_t C = g.geom.ncells;
     auto* st = s.status.as<pn::DevStatus>();
+    ++s.mail_seq;  // a new number for every binning enqueued, also one that fails below
+    s.mail_pending = false;
     hipLaunchKernelGGL(pn::k_reset_status, dim3(1), dim3(1), 0, g.stream, st);
     PN_CHECK_LAUNCH();
     hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(C, 1 << 20), 256)), dim3(256), 0,
@@ -322,7 +338,7 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
             PN_CHECK_LAUNCH();
         }
         int rc2 = s.scan.run(s.counts.as<int32_t>(), C, s.start.as<int32_t>(), s.nonempty.as<int32_t>(), st,
-                             g.stream);
+                             g.stream, s.mailbox, s.mail_seq);
         if (rc2) return rc2;
         if (sel_count > 0) {
             hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
@@ -367,6 +383,7 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
         return PN_OK;
     });
     if (rc) return rc;
+    s.mail_pending = true;
     s.n_sel = sel_count;
     s.n_src = n_src;
     s.src = src;
@@ -415,11 +432,39 @@ int describe_point(pn_grid& g, const void* src, int64_t p) {
     return PN_OK;
 }
 
-// Synchronizes and turns the device status of a binning pass into the reference's errors.
-int check_set(pn_grid& g, BinSet& s, const int32_t* sel_list, int64_t sel_first, bool is_query) {
+// The status of the last binning of s.  The mailbox path spins on the sequence number
+// k_scan_top raises (the host returns while the rest of the build still runs; a kernel
+// error after it surfaces at the next synchronization, as for any enqueued work); after
+// STATUS_SPIN_US without it, or with no mailbox, the status is copied after a stream
+// synchronization.
+constexpr int64_t STATUS_SPIN_US = 5000;
+
+int wait_status(pn_grid& g, BinSet& s) {
+    if (s.mailbox && s.mail_pending) {
+        s.mail_pending = false;
+        const unsigned long long want = s.mail_seq;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned it = 1;; ++it) {
+            if (__atomic_load_n(&s.mailbox->seq, __ATOMIC_ACQUIRE) == want) {
+                std::memcpy(&s.stats, (const void*)&s.mailbox->status, sizeof(pn::DevStatus));
+                return PN_OK;
+            }
+            __builtin_ia32_pause();
+            if ((it & 255) == 0 &&
+                std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(STATUS_SPIN_US))
+                break;
+        }
+    }
     PN_HIP(hipMemcpyAsync(s.host_status, s.status.p, sizeof(pn::DevStatus), hipMemcpyDeviceToHost, g.stream));
     PN_HIP(hipStreamSynchronize(g.stream));
     s.stats = *s.host_status;
+    return PN_OK;
+}
+
+// Waits for the status of a binning pass and turns it into the reference's errors.
+int check_set(pn_grid& g, BinSet& s, const int32_t* sel_list, int64_t sel_first, bool is_query) {
+    int rc0 = wait_status(g, s);
+    if (rc0) return rc0;
     s.stats_known = true;
     if (s.stats.first_bad != ~0ull) {
         s.valid = false;

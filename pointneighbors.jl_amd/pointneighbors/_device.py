@@ -37,8 +37,14 @@ def as_coordinates(a, ndims: int, name: str = "coordinates"):
     return a
 
 
+_NP_DTYPES: dict = {}
+
+
 def np_dtype(t) -> np.dtype:
-    return np.dtype(str(t.dtype).replace("torch.", ""))
+    dt = _NP_DTYPES.get(t.dtype)
+    if dt is None:
+        dt = _NP_DTYPES[t.dtype] = np.dtype(str(t.dtype).replace("torch.", ""))
+    return dt
 
 
 def selection(sel, n: int, device):
@@ -72,7 +78,12 @@ def is_full(sel, n: int) -> bool:
 
 
 def current_stream_handle(device) -> int:
+    """The raw HIP stream of torch's current stream on `device`."""
     torch = _torch()
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        return raw(idx)
     return torch.cuda.current_stream(device).cuda_stream
 
 

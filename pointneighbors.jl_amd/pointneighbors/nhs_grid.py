@@ -142,6 +142,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
                 raise ArgumentError(f"a {ndims}D PeriodicBox is required")
             self.n_cells, self.cell_size = periodic_grid_size(periodic_box, search_radius, r_dt)
         self._handle = None
+        self._stream_handle = None  # HIP stream last given to the handle
         self._coord_dtype = None
         self._built = False
         self._empty = False     # initialized with search_radius < eps()
@@ -179,6 +180,7 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
         if self._handle is not None:
             lib.pn_grid_destroy(self._handle)
             self._handle = None
+        self._stream_handle = None
         cl = self.cell_list
         nd = self.ndims
         if self._hashed():
@@ -234,7 +236,10 @@ class GridNeighborhoodSearch(metaclass=_GridMeta):
             check(load().pn_set_tuning(self._handle, key.encode(), int(value)), "set_tuning")
 
     def _set_stream(self, t):
-        load().pn_grid_set_stream(self._handle, ctypes.c_void_p(_device.current_stream_handle(t.device)))
+        h = _device.current_stream_handle(t.device)
+        if h != self._stream_handle:
+            load().pn_grid_set_stream(self._handle, ctypes.c_void_p(h))
+            self._stream_handle = h
 
     def host_cell_coords(self, coords):
         """cell_coords (src/nhs_grid.jl:622-628 with src/cell_lists/full_grid.jl:84-94) of

@@ -186,6 +186,37 @@ def test_overflow_error(golden):
         pn.initialize(nhs, d, d)
 
 
+def test_status_of_each_update(orc, golden):
+    """update! raises the errors of its own binning and no other: valid, NaN, overfull,
+    deferred and valid again in a row on one search (each binning publishes its status
+    under a new sequence number), and the last valid one counts like the oracle."""
+    msg = re.escape(golden["overflow"]["message"])
+    msg_bounds = re.escape(golden["bounds_errors"]["message"])
+    pts = cloud((12, 11), 3)
+    r = 1.6
+    nhs = make_nhs(pts, r, pts.min(axis=0) - r, pts.max(axis=0) + r, cap=20)
+    x = dev(pts)
+    pn.initialize(nhs, x, x)
+    assert nhs.stats()[0] == len(pts)
+    bad = pts.copy()
+    bad[7, 1] = np.nan
+    crowd = pts.copy()
+    crowd[:40] = pts[60]
+    for _ in range(3):
+        with pytest.raises(pn.PointNeighborsError, match=msg_bounds):
+            pn.update(nhs, dev(bad), dev(bad))
+        with pytest.raises(pn.PointNeighborsError, match=msg):
+            pn.update(nhs, dev(crowd), dev(crowd))
+        nhs.update(dev(crowd), dev(crowd), check_errors=False)
+        with pytest.raises(pn.PointNeighborsError, match=msg):
+            nhs.check()
+        pn.update(nhs, x, x)
+        assert nhs.stats()[0] == len(pts)
+    og = oracle_like(orc, nhs, np.float64)
+    assert og.build(pts)[0] == orc.OK
+    assert np.array_equal(gpu_counts(nhs, x, x), og.traverse(pts, pts)[2])
+
+
 def test_query_bounds_error():
     pts = cloud((10, 10), 1)
     nhs = make_nhs(pts, 2.5)

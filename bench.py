@@ -122,28 +122,16 @@ def main():
     f_rho = sph.SPHDensity(rho, mass, float(h), "cubic_spline", accumulate=False)
     f_dv = sph.SPHPressureAcceleration(dv, mass, P, float(h), "cubic_spline", accumulate=False)
 
-    def step(ev=None):
+    def step():
         disp.uniform_(-DISP, DISP, generator=gen)
         x.add_(disp.t())
         pn.update(nhs, x, x)
         if args.workload == "count":
-            if ev is not None:
-                ev[0].record(stream)
             pn.foreach_point_neighbor(f_count, x, x, nhs)
-            if ev is not None:
-                ev[1].record(stream)
             return
-        if ev is not None:
-            ev[0].record(stream)
         pn.foreach_point_neighbor(f_rho, x, x, nhs)
-        if ev is not None:
-            ev[1].record(stream)
         sph.tait_pressure_term(rho, RHO0, SOUND, GAMMA, out=P)
-        if ev is not None:
-            ev[2].record(stream)
         pn.foreach_point_neighbor(f_dv, x, x, nhs)
-        if ev is not None:
-            ev[3].record(stream)
 
     pn.initialize(nhs, x, x)
     tuned = None
@@ -162,18 +150,16 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    nev = 2 if args.workload == "count" else 4
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(nev)] for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     # each traversal kernel's duration: HIP events the library records on its launch stream
-    # right around the kernel (the events above also hold the host's launch latency)
+    # right around the kernel
     nhs.set_tuning("time_launches", 1)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     t_start.record(stream)
-    for k in range(args.steps):
-        step(ev[k])
+    for _ in range(args.steps):
+        step()
     t_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
@@ -198,7 +184,14 @@ def main():
         extra["count_traversal_ms"] = trav_ms
     else:
         dens_ms = float(np.mean(launch_ms[0::2]))
-        eos_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+        # the equation of state alone, after the timed steps
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(20):
+            sph.tait_pressure_term(rho, RHO0, SOUND, GAMMA, out=P)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        eos_ms = e0.elapsed_time(e1) / 20
         pres_ms = float(np.mean(launch_ms[1::2]))
         extra.update({"density_traversal_ms": dens_ms, "eos_ms": eos_ms, "pressure_traversal_ms": pres_ms,
                       "density_pairs_per_s": pairs / (dens_ms * 1e-3),

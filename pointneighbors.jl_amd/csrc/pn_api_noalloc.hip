@@ -156,7 +156,7 @@ int pn_count_neighbors_noalloc(int ndims, int coord_dtype, int radius_dtype, int
                            (pn::StatusMailbox*)nullptr, 0ull);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scan_down, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, s, (const int32_t*)cnt, C,
-                           (const unsigned long long*)bs, start, nonempty);
+                           (const unsigned long long*)bs, start, nonempty, (int32_t*)nullptr);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const int32_t*)keys,
                            (const int32_t*)slots, n, (const int32_t*)start, perm_tmp);

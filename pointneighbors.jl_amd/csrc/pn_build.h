@@ -414,10 +414,11 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(unsigned long 
     }
 }
 
-static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t* __restrict__ counts, int64_t C,
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t* counts, int64_t C,
                                                             const unsigned long long* __restrict__ block_sums,
                                                             int32_t* __restrict__ start,
-                                                            int32_t* __restrict__ nonempty) {
+                                                            int32_t* __restrict__ nonempty,
+                                                            int32_t* clear) {
     __shared__ unsigned long long sh[SCAN_THREADS / WAVE];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     int32_t v[SCAN_ITEMS];
@@ -427,6 +428,12 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t
         int64_t c = base + i;
         v[i] = c < C ? counts[c] : 0;
         s += pack_cell(v[i]);
+    }
+    // clear = counts: the counts are read once, here, and left zero for the next binning
+    if (clear) {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < C) clear[base + i] = 0;
     }
     unsigned long long total;
     unsigned long long run = block_sums[blockIdx.x] + block_excl_scan(s, sh, total);

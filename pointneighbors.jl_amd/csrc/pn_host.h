@@ -77,7 +77,8 @@ struct Buf {
 struct Scanner {
     Buf block_sums;
     int run(const int32_t* counts, int64_t n, int32_t* start, int32_t* nonempty, pn::DevStatus* st,
-            hipStream_t stream, pn::StatusMailbox* mb = nullptr, unsigned long long seq = 0) {
+            hipStream_t stream, pn::StatusMailbox* mb = nullptr, unsigned long long seq = 0,
+            bool clear_counts = false) {
         const int64_t nb = std::max<int64_t>(1, (n + pn::SCAN_TILE - 1) / pn::SCAN_TILE);
         int rc = block_sums.ensure((size_t)nb * 16, stream);  // block sums, then block maxima
         if (rc) return rc;
@@ -89,7 +90,8 @@ struct Scanner {
                            seq);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scan_down, dim3((unsigned)nb), dim3(pn::SCAN_THREADS), 0, stream, counts, n,
-                           (const unsigned long long*)bs, start, nonempty);
+                           (const unsigned long long*)bs, start, nonempty,
+                           clear_counts ? const_cast<int32_t*>(counts) : nullptr);
         PN_CHECK_LAUNCH();
         return PN_OK;
     }
@@ -107,6 +109,7 @@ struct BinSet {
     unsigned long long mail_seq = 0;
     bool mail_pending = false;
     Scanner scan;
+    bool counts_zero = false;  // `counts` holds zeros (left so by the last binning's scan)
     int64_t cap = 0;
     int64_t n_sel = 0;        // selected points of the last binning
     int64_t n_src = 0;        // columns of the source matrix
@@ -243,7 +246,9 @@ int dispatch_trav(const pn_grid& g, Fn&& fn) {
 int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
     const int64_t C = g.geom.ncells;
     int rc;
+    const void* counts_before = s.counts.p;
     if ((rc = s.counts.ensure((size_t)C * 4, g.stream))) return rc;
+    if (s.counts.p != counts_before) s.counts_zero = false;
     if ((rc = s.start.ensure((size_t)(C + 1) * 4, g.stream))) return rc;
     if ((rc = s.nonempty.ensure((size_t)C * 4, g.stream))) return rc;
     if ((rc = s.status.ensure(sizeof(pn::DevStatus), g.stream))) return rc;
@@ -324,9 +329,12 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
     s.mail_pending = false;
     hipLaunchKernelGGL(pn::k_reset_status, dim3(1), dim3(1), 0, g.stream, st);
     PN_CHECK_LAUNCH();
-    hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(C, 1 << 20), 256)), dim3(256), 0,
-                       g.stream, s.counts.as<int32_t>(), C, 0);
-    PN_CHECK_LAUNCH();
+    if (!s.counts_zero) {
+        hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(C, 1 << 20), 256)), dim3(256), 0,
+                           g.stream, s.counts.as<int32_t>(), C, 0);
+        PN_CHECK_LAUNCH();
+    }
+    s.counts_zero = false;  // until this binning's scan is enqueued
     rc = dispatch_build(g, [&](auto dc, auto tcv, auto tcellv) -> int {
         constexpr int D = decltype(dc)::value;
         using TC = decltype(tcv);
@@ -338,8 +346,9 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
             PN_CHECK_LAUNCH();
         }
         int rc2 = s.scan.run(s.counts.as<int32_t>(), C, s.start.as<int32_t>(), s.nonempty.as<int32_t>(), st,
-                             g.stream, s.mailbox, s.mail_seq);
+                             g.stream, s.mailbox, s.mail_seq, true);
         if (rc2) return rc2;
+        s.counts_zero = true;
         if (sel_count > 0) {
             hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
                                s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_count, s.start.as<int32_t>(),

@@ -120,9 +120,12 @@ int pn_grid_set_stream(pn_grid_t grid, void* stream);
  *   y         device pointer, D x n_y of coord_dtype
  *   sel_list  NULL: the selection is the range [sel_first, sel_first + sel_count);
  *             else a device int32 array of sel_count 0-based indices (eachindex_y)
- * pn_grid_update synchronizes and reports the reference's errors:
+ * pn_grid_update waits for the binning's status and reports the reference's errors:
  *   PN_ERR_OUT_OF_BOUNDS (first offending point via pn_grid_error), PN_ERR_CELL_FULL.
- * pn_grid_update_async only enqueues; pn_grid_check later synchronizes and reports.
+ *   The status reaches pinned host memory from the scan, before the last build kernels end:
+ *   the call returns while they still run on the grid's stream (later work on that stream
+ *   is ordered after them; other streams or the host synchronize with it first).
+ * pn_grid_update_async only enqueues; pn_grid_check later waits and reports.
  * ---------------------------------------------------------------------------------- */
 int pn_grid_update(pn_grid_t grid, const void* y, int64_t n_y, int64_t sel_first,
                    int64_t sel_count, const int32_t* sel_list);

@@ -68,18 +68,24 @@ int pn_wcsph_interact(pn_grid_t grid, int kernel, double h, double sound_speed, 
         f.dvmat = (T*)dv;
         f.accumulate = accumulate;
         const T* vm = (const T*)v;
-        // fields in sorted order: m_b, rho_b, p_b, v_b
+        // fields in sorted order: m_b, rho_b, p_b, (Float64: 1 / rho_b,) v_b
+        using F = pn::WCSPHInteractF<T, D>;
         pn::FieldSrc<T> src{};
-        src.n = 3 + D;
+        src.n = F::kFields;
         src.p[0] = (const T*)mass;
         src.step[0] = 1;
         src.p[1] = vm + D;
         src.step[1] = D + 1;
         src.p[2] = (const T*)pressure;
         src.step[2] = 1;
+        if (F::kInvRho) {
+            src.p[3] = vm + D;
+            src.step[3] = D + 1;
+            src.recip = 1u << 3;
+        }
         for (int d = 0; d < D; ++d) {
-            src.p[3 + d] = vm + d;
-            src.step[3 + d] = D + 1;
+            src.p[F::kV + d] = vm + d;
+            src.step[F::kV + d] = D + 1;
         }
         return launch_traverse_t<D, T, T>(g, Q, f, &src);
     });

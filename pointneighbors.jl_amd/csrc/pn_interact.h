@@ -73,14 +73,17 @@ struct WCSPHParams {
 };
 
 // WCSPH fluid-fluid interaction.  Fields (per neighbour, sorted order): m_b, rho_b, p_b,
-// v_b (D values).  Query state: v_a, rho_a, p_a (reloaded where a flush needs it).
+// in Float64 1 / rho_b (divided once per point by the field gather instead of once per
+// pair), v_b (D values).  Query state: v_a, rho_a, p_a (reloaded where a flush needs it).
 template <class TR, int D>
 struct WCSPHInteractF : FunctorBase {
     struct Acc {
         TR v[D + 1];  // dv_a, then drho_a
     };
     static constexpr bool kQueue = true;
-    static constexpr int kFields = 3 + D;
+    static constexpr bool kInvRho = sizeof(TR) == 8;
+    static constexpr int kV = kInvRho ? 4 : 3;  // field of v_b's first component
+    static constexpr int kFields = kV + D;
     struct QState {
         TR v[D];
         TR rho, p, inv_rho;
@@ -115,8 +118,8 @@ struct WCSPHInteractF : FunctorBase {
         const T gw = dist < sqrt_eps<T>() ? (T)0 : kern.dw(dist) * inv_d;  // gradW = gw * pos_diff
         T vr = (T)0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) vr = pfma(q.v[d] - fj[3 + d], pd[d], vr);
-        const T inv_rhob = prcp(rhob);
+        for (int d = 0; d < D; ++d) vr = pfma(q.v[d] - fj[kV + d], pd[d], vr);
+        const T inv_rhob = kInvRho ? fj[3] : prcp(rhob);
         // pressure and viscosity (the viscosity only for approaching pairs, selected per lane);
         // 1 / rho_mean = 2 / (rho_a + rho_b)
         T coef = -mb * (q.p + pb) * q.inv_rho * inv_rhob;

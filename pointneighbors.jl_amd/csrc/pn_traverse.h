@@ -501,13 +501,14 @@ __global__ __launch_bounds__(WAVE) void k_traverse(Geom g, SetView Q, SetView G,
 
 // Gathers per-neighbour fields into the grid's sorted order: out[f * stride + k] =
 // src_f[sorted_idx[k] * step_f + off_f] (a field may be one component of a D x N array).
-constexpr int MAX_FIELDS = 6;
+constexpr int MAX_FIELDS = 7;
 
 template <class TR>
 struct FieldSrc {
     const TR* p[MAX_FIELDS];
     int step[MAX_FIELDS];
     int n;
+    unsigned recip;  // bit f set: field f is gathered as 1 / value (an IEEE division)
 };
 
 template <class TR>
@@ -517,8 +518,12 @@ __global__ __launch_bounds__(256) void k_gather_fields(const int32_t* __restrict
     if (k >= st->n_binned) return;
     const int64_t p = sorted_idx[k];
 #pragma unroll
-    for (int f = 0; f < MAX_FIELDS; ++f)
-        if (f < src.n) out[(int64_t)f * stride + k] = src.p[f][p * src.step[f]];
+    for (int f = 0; f < MAX_FIELDS; ++f) {
+        if (f < src.n) {
+            const TR v = src.p[f][p * src.step[f]];
+            out[(int64_t)f * stride + k] = ((src.recip >> f) & 1u) ? (TR)1 / v : v;
+        }
+    }
 }
 
 }  // namespace pn

@@ -38,7 +38,21 @@ __device__ __forceinline__ float psqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ double psqrt(double x) { return sqrt(x); }
 // 1/sqrt(x) for x > 0, 0 for x == 0
 __device__ __forceinline__ float prsqrt0(float x) { return x > 0.0f ? __builtin_amdgcn_rsqf(x) : 0.0f; }
-__device__ __forceinline__ double prsqrt0(double x) { return x > 0.0 ? 1.0 / sqrt(x) : 0.0; }
+// Float64: the hardware estimate (v_rsq_f64) refined by two Newton steps
+// y <- y + y (1 - x y^2) / 2 in FMAs, within an ulp or so of the rounded 1/sqrt(x) at a third
+// of the cost of a correctly rounded square root and a division; x outside [2^-1000, 2^1000]
+// (denormals included) takes the division.
+__device__ __forceinline__ double prsqrt0(double x) {
+    if (!(x > 0.0)) return 0.0;
+    if (x < 0x1p-1000 || x > 0x1p+1000) return 1.0 / sqrt(x);
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    return y;
+}
 // sqrt(eps(T)) in T, Julia's threshold below which a distance counts as zero
 template <class T>
 __device__ __forceinline__ T sqrt_eps();

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "pn_build.h"
+#include "pn_sph.h"
 
 namespace pn {
 
@@ -388,10 +389,12 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
             T d2 = Xab[0] * Xab[0];
 #pragma unroll
             for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
-            const T dist = sqrt(d2);
+            // r and 1/r from one reciprocal square root
+            const T inv = prsqrt0(d2);
+            const T dist = d2 * inv;
             const bool use = j[u] >= 0 && dist >= tiny;
             // V_b W'(r) / r, zero for pairs that do not contribute
-            const T s = use ? vj[u] * kern.dw(dist) / dist : (T)0;
+            const T s = use ? vj[u] * kern.dw(dist) * inv : (T)0;
 #pragma unroll
             for (int r = 0; r < D; ++r) {
                 const T dx = xj[u][r] - xi[r];
@@ -537,11 +540,13 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restr
             T d2 = Xab[0] * Xab[0];
 #pragma unroll
             for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
-            const T dist = sqrt(d2);
+            const T inv = prsqrt0(d2);
+            const T dist = d2 * inv;
             const bool use = j[u] >= 0 && dist >= tiny;
             const T safe = use ? dist : (T)1;
+            const T isafe = use ? inv : (T)1;
             // m_b W'(r) / r, so that m_b grad_a W = s X_ab
-            const T s = use ? mj[u] * kern.dw(safe) / safe : (T)0;
+            const T s = use ? mj[u] * kern.dw(safe) * isafe : (T)0;
 #pragma unroll
             for (int r = 0; r < D; ++r) {
                 T v = (T)0;
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restr
                     dl = fma(eps[d], xab[d], dl);
                 }
                 // V_b W(r) / r^2 * delta / |x_ab| * (alpha / 2) E / rho_a, times x_ab / |x_ab|
-                const T g = use ? pen_a * (mj[u] / rj[u]) * kern.w(safe) / (safe * safe) * dl / c2 : (T)0;
+                const T g = use ? pen_a * (mj[u] / rj[u]) * kern.w(safe) * (isafe * isafe) * dl / c2 : (T)0;
 #pragma unroll
                 for (int d = 0; d < D; ++d) acc[d] = fma(g, xab[d], acc[d]);
             }
@@ -670,10 +675,12 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force_rec(const int32_t* __r
         T d2 = Xab[0] * Xab[0];
 #pragma unroll
         for (int d = 1; d < D; ++d) d2 = d2 + Xab[d] * Xab[d];
-        const T dist = sqrt(d2);
+        const T inv = prsqrt0(d2);
+        const T dist = d2 * inv;
         const bool use = dist >= tiny;
         const T safe = use ? dist : (T)1;
-        const T s = use ? rj[RC::M] * kern.dw(safe) / safe : (T)0;
+        const T isafe = use ? inv : (T)1;
+        const T s = use ? rj[RC::M] * kern.dw(safe) * isafe : (T)0;
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             T v = (T)0;
@@ -697,7 +704,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force_rec(const int32_t* __r
             c2 = c2 + xab[d] * xab[d];
             dl = fma(eps[d], xab[d], dl);
         }
-        const T g = use ? pen_a * rj[RC::V] * kern.w(safe) / (safe * safe) * dl / c2 : (T)0;
+        const T g = use ? pen_a * rj[RC::V] * kern.w(safe) * (isafe * isafe) * dl / c2 : (T)0;
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d] = fma(g, xab[d], acc[d]);
     }

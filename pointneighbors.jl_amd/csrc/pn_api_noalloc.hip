@@ -159,7 +159,7 @@ int pn_count_neighbors_noalloc(int ndims, int coord_dtype, int radius_dtype, int
                            (const unsigned long long*)bs, start, nonempty, (int32_t*)nullptr);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const int32_t*)keys,
-                           (const int32_t*)slots, n, (const int32_t*)start, perm_tmp);
+                           (const int32_t*)slots, n, (const int32_t*)start, perm_tmp, (int32_t*)nullptr);
         PN_CHECK_LAUNCH();
         hipLaunchKernelGGL((pn::k_canon<D, TC>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const int32_t*)keys,
                            (int64_t)0, (const int32_t*)nullptr, (const pn::DevStatus*)st, (const int32_t*)start,

@@ -450,16 +450,19 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int32_t
 
 // Provisional placement: selection position a goes to slot start[key] + slots[a] of its
 // cell segment (atomic order).  The slot holds a, not the point index, so k_canon can
-// recover the key and break ties between repeated selections.
-static __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ keys,
-                                                 const int32_t* __restrict__ slots, int64_t n_sel,
-                                                 const int32_t* __restrict__ start,
-                                                 int32_t* __restrict__ perm_tmp) {
+// recover the key and break ties between repeated selections.  With reset = slots, each
+// entry is set to -1 once read (the hash build reuses the array, cleared, for its bucket
+// representatives).
+static __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ keys, const int32_t* slots,
+                                                 int64_t n_sel, const int32_t* __restrict__ start,
+                                                 int32_t* __restrict__ perm_tmp, int32_t* reset) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_sel) return;
     const int key = keys[a];
+    const int slot = key < 0 ? 0 : slots[a];
+    if (reset) reset[a] = -1;
     if (key < 0) return;
-    perm_tmp[start[key] + slots[a]] = (int32_t)a;
+    perm_tmp[start[key] + slot] = (int32_t)a;
 }
 
 // Cells with more points than this keep the provisional (atomic) order instead of the

@@ -350,9 +350,10 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
         if (rc2) return rc2;
         s.counts_zero = true;
         if (sel_count > 0) {
+            // a hash build reuses `slots`, cleared to -1 here, for its bucket representatives
             hipLaunchKernelGGL(pn::k_scatter, dim3(blocks_for(sel_count, 256)), dim3(256), 0, g.stream,
                                s.keys.as<int32_t>(), s.slots.as<int32_t>(), sel_count, s.start.as<int32_t>(),
-                               s.perm_tmp.as<int32_t>());
+                               s.perm_tmp.as<int32_t>(), g.geom.hash ? s.slots.as<int32_t>() : nullptr);
             PN_CHECK_LAUNCH();
             // one thread per binned slot (at most sel_count; the count is read on the device)
             if (!g.geom.hash) {
@@ -371,11 +372,8 @@ int enqueue_bin(pn_grid& g, BinSet& s, const void* src, int64_t n_src, int64_t s
                     s.hash_epoch = 0;
                 }
                 ++s.hash_epoch;
-                // bucket representatives by selection position, in `slots` (free after k_scatter)
+                // bucket representatives by selection position, in `slots` (-1 after k_scatter)
                 int32_t* rep = s.slots.as<int32_t>();
-                hipLaunchKernelGGL(pn::k_fill_i32, dim3(blocks_for(std::min<int64_t>(sel_count, 1 << 20), 256)),
-                                   dim3(256), 0, g.stream, rep, sel_count, -1);
-                PN_CHECK_LAUNCH();
                 hipLaunchKernelGGL((pn::k_canon_hash<D, TC, TCELL>), dim3(blocks_for(sel_count, 256)), dim3(256), 0,
                                    g.stream, s.keys.as<int32_t>(), sel_first, sel_list, (const pn::DevStatus*)st,
                                    s.start.as<int32_t>(), s.perm_tmp.as<int32_t>(), (const TC*)src,

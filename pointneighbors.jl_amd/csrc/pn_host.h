@@ -108,6 +108,7 @@ struct BinSet {
     pn::StatusMailbox* mailbox = nullptr;
     unsigned long long mail_seq = 0;
     bool mail_pending = false;
+    bool mailbox_failed = false;  // its allocation failed once: not tried again
     Scanner scan;
     bool counts_zero = false;  // `counts` holds zeros (left so by the last binning's scan)
     int64_t cap = 0;
@@ -259,11 +260,17 @@ int ensure_set(pn_grid& g, BinSet& s, int64_t n_sel) {
         if (s.bucket_mixed.bytes != before) s.hash_epoch = 0;  // new memory: cleared before use
     }
     if (!s.host_status) PN_HIP(hipHostMalloc((void**)&s.host_status, sizeof(pn::DevStatus), hipHostMallocDefault));
-    if (!s.mailbox) {
-        PN_HIP(hipHostMalloc((void**)&s.mailbox, sizeof(pn::StatusMailbox),
-                             hipHostMallocMapped | hipHostMallocCoherent));
-        std::memset((void*)s.mailbox, 0, sizeof(pn::StatusMailbox));
-        s.mail_seq = 0;
+    if (!s.mailbox && !s.mailbox_failed) {
+        // optional: without it the status is copied after a stream synchronization
+        if (hipHostMalloc((void**)&s.mailbox, sizeof(pn::StatusMailbox),
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            (void)hipGetLastError();
+            s.mailbox = nullptr;
+            s.mailbox_failed = true;
+        } else {
+            std::memset((void*)s.mailbox, 0, sizeof(pn::StatusMailbox));
+            s.mail_seq = 0;
+        }
     }
     if (n_sel > s.cap) {
         int64_t cap = std::max<int64_t>(n_sel, 64);
@@ -456,7 +463,9 @@ int wait_status(pn_grid& g, BinSet& s) {
                 std::memcpy(&s.stats, (const void*)&s.mailbox->status, sizeof(pn::DevStatus));
                 return PN_OK;
             }
+#if defined(__x86_64__) || defined(__i386__)
             __builtin_ia32_pause();
+#endif
             if ((it & 255) == 0 &&
                 std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(STATUS_SPIN_US))
                 break;

@@ -426,7 +426,6 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
         for (int d = 0; d < ndims; ++d)
             if (!(box_size[d] > 0.0)) return fail(PN_ERR_ARGUMENT, "box sizes must be positive");
     if (sel_count <= 0) return PN_OK;
-    const int nblocks = (int)round_up8((sel_count + pn::LIST_ROWS_PER_BLOCK - 1) / pn::LIST_ROWS_PER_BLOCK);
     pn_grid tmp;  // dispatch only
     tmp.nd = ndims;
     tmp.tc = coord_dtype;
@@ -435,6 +434,8 @@ static int list_sph_density(const int32_t* row_ptr, const int32_t* col_idx, int6
         constexpr int D = decltype(dc)::value;
         using TC = decltype(tcv);
         using TR = decltype(trv);
+        constexpr int rows = pn::ListShape<TR, false>::rows;
+        const int nblocks = (int)round_up8((sel_count + rows - 1) / rows);
         auto kern = pn::SmoothingKernel<TR, D>::make(kernel, h);
         pn::ListPeriodic<TR> per{};
         per.on = box_size ? 1 : 0;
@@ -481,10 +482,11 @@ int pn_list_tlsph_deformation_grad(const int32_t* row_ptr, const int32_t* col_id
     if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
         return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
     if (sel_count <= 0) return PN_OK;
-    const int nblocks = (int)round_up8((sel_count + pn::LIST_ROWS_PER_BLOCK - 1) / pn::LIST_ROWS_PER_BLOCK);
     auto run = [&](auto dc, auto tv) -> int {
         constexpr int D = decltype(dc)::value;
         using T = decltype(tv);
+        constexpr int rows = pn::ListShape<T, true>::rows;
+        const int nblocks = (int)round_up8((sel_count + rows - 1) / rows);
         auto kern = pn::SmoothingKernel<T, D>::make(kernel, h);
         hipLaunchKernelGGL((pn::k_list_tlsph_grad<D, T, pn::SmoothingKernel<T, D>>), dim3(nblocks), dim3(256), 0,
                            (hipStream_t)stream, row_ptr, col_idx, stored_base, sel_first, sel_list, sel_count,
@@ -515,11 +517,12 @@ int pn_list_tlsph_interaction(const int32_t* row_ptr, const int32_t* col_idx, in
     if (!sel_list && (sel_first < 0 || sel_first + sel_count > n_rows))
         return fail(PN_ERR_ARGUMENT, "selection range out of bounds of the neighbor lists");
     if (sel_count <= 0) return PN_OK;
-    const int nblocks = (int)round_up8((sel_count + pn::LIST_ROWS_PER_BLOCK - 1) / pn::LIST_ROWS_PER_BLOCK);
     auto run = [&](auto dc, auto tv) -> int {
         constexpr int D = decltype(dc)::value;
         using T = decltype(tv);
         using K = pn::SmoothingKernel<T, D>;
+        constexpr int rows = pn::ListShape<T, true>::rows;
+        const int nblocks = (int)round_up8((sel_count + rows - 1) / rows);
         auto kern = K::make(kernel, h);
         const T hae = (T)(0.5 * penalty_alpha * young_modulus);
         if (pen && workspace) {

@@ -251,14 +251,24 @@ struct ListPeriodic {
 
 // SPH density over stored lists (the TLSPH-style sum of a PrecomputedNeighborhoodSearch,
 // src/nhs_precomputed.jl:210-247: stored neighbours, distance recomputed, no radius
-// test).  Sixteen lanes per selected row (LIST_ROWS_PER_BLOCK rows per 256-thread block):
-// each lane first loads up to LIST_UNROLL of its row's column indices, then all their
-// coordinates and masses, so a row's loads are in flight together; the 16 partial sums are
-// combined in a fixed butterfly order.  x and y are D x N coordinate matrices (original
+// test).  ListShape lanes per selected row (256 / lanes rows per 256-thread block): each
+// lane first loads up to LIST_UNROLL of its row's column indices, then all their
+// coordinates and masses, so a row's loads are in flight together; the lanes' partial sums
+// are combined in a fixed butterfly order.  x and y are D x N coordinate matrices (original
 // order), mass per y point, rho per x point.
-constexpr int LIST_LANES = 16;
-constexpr int LIST_ROWS_PER_BLOCK = 256 / LIST_LANES;
 constexpr int LIST_UNROLL = 4;  // measured best of 1, 2, 4, 8 for the density sum
+
+// Lanes per row of the list kernels (a power of two dividing 64) and rows per 256-thread
+// block.  Fewer lanes per row put more rows in flight and shorten the closing butterfly: 8
+// measured faster for the density sum (0.555 vs 0.588 ms on the 128^3 lists) and for every
+// Float64 TLSPH kernel (correction matrix 0.51 vs 0.57 ms, deformation gradient 0.58 vs 0.60,
+// interaction 1.745 vs 1.79), 16 for the Float32 TLSPH kernels (deformation gradient 0.384
+// vs 0.409 ms, interaction 0.952 vs 0.965).
+template <class T, bool TLSPH>
+struct ListShape {
+    static constexpr int lanes = (TLSPH && sizeof(T) == 4) ? 16 : 8;
+    static constexpr int rows = 256 / lanes;
+};
 
 template <int D, class TC, class TR, class K>
 __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict__ row_ptr,
@@ -268,9 +278,11 @@ __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict_
                                                       const TC* __restrict__ y, const TR* __restrict__ mass,
                                                       K kern, TR* __restrict__ rho, int accumulate, int nblocks,
                                                       ListPeriodic<TR> per) {
+    constexpr int LIST_LANES = ListShape<TR, false>::lanes;
+    constexpr int LIST_ROWS_PER_BLOCK = ListShape<TR, false>::rows;
     const int sub = threadIdx.x & (LIST_LANES - 1);
     const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
-    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    if (a >= n_sel) return;  // a row's lanes leave together
     const int64_t i = sel_point(a, sel_first, sel_list);
     const int b = row_ptr[i], e = row_ptr[i + 1];
     TC xi[D];
@@ -334,9 +346,9 @@ __global__ __launch_bounds__(256) void k_list_density(const int32_t* __restrict_
 //     grad_a W(X_ab) = W'(|X_ab|) X_ab / |X_ab|,   X_ab = X_a - X_b,
 // out_a = G_a L_a (L given), G_a^-1 (invert; closed-form adjugate over determinant) or G_a.
 // With x = X this is the (symmetric) kernel-gradient correction matrix M_a, and L = M^-1
-// (invert) makes G reproduce affine maps exactly.  Sixteen lanes
+// (invert) makes G reproduce affine maps exactly.  ListShape lanes
 // per row as in k_list_density; each lane keeps its D x D partial sum, the butterfly over the
-// 16 lanes has a fixed order.  X, x: D x N; out and L: D x D per point, column-major.
+// lanes has a fixed order.  X, x: D x N; out and L: D x D per point, column-major.
 template <int D, class T, class K>
 __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restrict__ row_ptr,
                                                          const int32_t* __restrict__ col, int stored_base,
@@ -344,9 +356,11 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
                                                          int64_t n_sel, const T* __restrict__ X, const T* __restrict__ x,
                                                          const T* __restrict__ vol, K kern, const T* __restrict__ L,
                                                          int invert, T* __restrict__ out, int nblocks) {
+    constexpr int LIST_LANES = ListShape<T, true>::lanes;
+    constexpr int LIST_ROWS_PER_BLOCK = ListShape<T, true>::rows;
     const int sub = threadIdx.x & (LIST_LANES - 1);
     const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
-    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    if (a >= n_sel) return;  // a row's lanes leave together
     const int64_t i = sel_point(a, sel_first, sel_list);
     const int b = row_ptr[i], e = row_ptr[i + 1];
     T Xi[D], xi[D];
@@ -474,7 +488,7 @@ __global__ __launch_bounds__(256) void k_list_tlsph_grad(const int32_t* __restri
 // enabled when PEN) with x_ab = x_a - x_b the current difference, F the deformation gradient:
 //     delta = ((F_a + F_b) X_ab - 2 x_ab) . x_ab / |x_ab|
 //     f_ab  = (alpha / 2) V_a V_b W(|X_ab|) / |X_ab|^2 * E * delta * x_ab / |x_ab|,  V = m / rho.
-// Both terms are antisymmetric in (a, b): sum_a m_a dv_a = 0 over symmetric lists.  Sixteen
+// Both terms are antisymmetric in (a, b): sum_a m_a dv_a = 0 over symmetric lists.  ListShape
 // lanes per row as in k_list_density; dv: D x N.
 template <int D, class T, class K, bool PEN>
 __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restrict__ row_ptr,
@@ -485,9 +499,11 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force(const int32_t* __restr
                                                           const T* __restrict__ rho, const T* __restrict__ Pr,
                                                           const T* __restrict__ F, T half_alpha_E, K kern,
                                                           T* __restrict__ dv, int accumulate, int nblocks) {
+    constexpr int LIST_LANES = ListShape<T, true>::lanes;
+    constexpr int LIST_ROWS_PER_BLOCK = ListShape<T, true>::rows;
     const int sub = threadIdx.x & (LIST_LANES - 1);
     const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
-    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    if (a >= n_sel) return;  // a row's lanes leave together
     const int64_t i = sel_point(a, sel_first, sel_list);
     const int b = row_ptr[i], e = row_ptr[i + 1];
     T Xi[D], xi[D], Pi[D * D], Fi[PEN ? D * D : 1];
@@ -654,9 +670,11 @@ __global__ __launch_bounds__(256) void k_list_tlsph_force_rec(const int32_t* __r
                                                               const T* __restrict__ rho, T half_alpha_E, K kern,
                                                               T* __restrict__ dv, int accumulate, int nblocks) {
     using RC = TlsphRec<D, T>;
+    constexpr int LIST_LANES = ListShape<T, true>::lanes;
+    constexpr int LIST_ROWS_PER_BLOCK = ListShape<T, true>::rows;
     const int sub = threadIdx.x & (LIST_LANES - 1);
     const int64_t a = (int64_t)xcd_remap(blockIdx.x, nblocks) * LIST_ROWS_PER_BLOCK + (threadIdx.x / LIST_LANES);
-    if (a >= n_sel) return;  // whole 16-lane groups leave together
+    if (a >= n_sel) return;  // a row's lanes leave together
     const int64_t i = sel_point(a, sel_first, sel_list);
     const int b = row_ptr[i], e = row_ptr[i + 1];
     T ri[RC::R];

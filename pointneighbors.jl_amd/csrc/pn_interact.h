@@ -202,8 +202,8 @@ struct MapReduceF : FunctorBase {
 
 // ------------------------------------------------------------------------------------
 // n-body over precomputed lists (src/nhs_precomputed.jl:210-247: stored neighbours,
-// distance recomputed, no radius test).  Sixteen lanes per selected row, as k_list_density;
-// the 16 partial sums are combined in a fixed butterfly order.
+// distance recomputed, no radius test).  Sixteen lanes per selected row; the 16 partial
+// sums are combined in a fixed butterfly order.
 // ------------------------------------------------------------------------------------
 template <int D, class TC, class TR>
 __global__ __launch_bounds__(256) void k_list_nbody(const int32_t* __restrict__ row_ptr,
